@@ -1,0 +1,330 @@
+/*
+ * svo_hip.h -- C-ABI of the MI355X serialized-SVO treelet ray caster.
+ *
+ * Plain C, plain pointers and sizes.  Every function returns an svo_status
+ * (0 = SVO_OK) and leaves a thread-local message for svo_last_error() on
+ * failure; nothing in the library calls exit().
+ *
+ * Handles are opaque and not thread-safe; device work is ordered on the
+ * hipStream_t passed in (as void*, NULL = the default stream).  Device pools
+ * are owned by the handles, frame outputs are owned by the caller.
+ *
+ * Reference interfaces replaced (paths relative to the libsvo tree):
+ *   svo_scene_*      Treelet / TreeletBuildManager / TreeletMemoryManager
+ *                    loadFromFile (src/Treelet.cpp:403-478,
+ *                    src/TreeletBuildManager.cpp:286-342,
+ *                    src/TreeletMemoryManager.cpp:229-291)
+ *   svo_mm_*         TreeletMemoryManager + TreeletMemoryManagerCl
+ *                    (src/TreeletMemoryManager.h:97-157,
+ *                    src/TreeletMemoryManagerCl.h:69-93)
+ *   svo_render       kernel renderToBuffer
+ *                    (opencl/treeletRenderer_rgba.cl:610-655) as enqueued by
+ *                    projects/raycast_serialized_opencl/src/
+ *                    raycast_serialized_opencl.cpp:729-749
+ */
+#ifndef SVO_HIP_H
+#define SVO_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ status */
+
+typedef int svo_status;
+enum {
+  SVO_OK = 0,
+  SVO_ERR_ARG = 1,      /* bad argument (null, out of range)               */
+  SVO_ERR_IO = 2,       /* file could not be opened / read / written       */
+  SVO_ERR_FORMAT = 3,   /* malformed .svo / .ia content                    */
+  SVO_ERR_HIP = 4,      /* a HIP runtime call failed                       */
+  SVO_ERR_NOMEM = 5,    /* host or device allocation failed                */
+  SVO_ERR_STATE = 6,    /* call not valid in the handle's current state    */
+  SVO_ERR_CAPACITY = 7  /* incore pool has no free / freeable slot         */
+};
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* svo_last_error(void);
+
+/* ------------------------------------------------------------ data model */
+
+/* One serialized octree node, 8 bytes, identical bit layout to
+ * CpuSvoNode (src/CpuSvoNode.h:46-48,100-102) and the OpenCL SvoNode
+ * (opencl/treeletRenderer_rgba.cl:15-19):
+ *   first_child : inner node -> offset of its first child RELATIVE to the
+ *                 node's own index (may be negative after stitching, kept
+ *                 modulo 2^32); treelet leaf -> gid of the sub-treelet that
+ *                 continues below it (0 = none).
+ *   masks       : bits 0-7 leaf mask, bits 8-15 valid mask, 16-31 zero.   */
+typedef struct svo_node {
+  uint32_t first_child;
+  uint32_t masks;
+} svo_node;
+
+/* Per-treelet header of the .svo bundle, in file order
+ * (src/Treelet.cpp:406-418). */
+typedef struct svo_treelet_header {
+  uint32_t num_nodes;
+  uint32_t num_leaves;
+  uint32_t mem_size;               /* bytes of one treelet slot          */
+  uint32_t gid;
+  uint32_t parent_gid;
+  uint32_t parent_leaf_pos;        /* leaf's local index in the parent   */
+  uint32_t parent_leaf_idx;        /* 0..7 child slot of that leaf       */
+  uint32_t parent_leaf_parent_pos; /* local index of the leaf's parent   */
+} svo_treelet_header;
+
+typedef struct svo_scene svo_scene;
+
+typedef struct svo_scene_info {
+  uint32_t num_treelets;
+  uint32_t treelet_bytes;       /* mem_size of every treelet            */
+  uint32_t nodes_per_treelet;   /* treelet_bytes / 8                    */
+  uint32_t max_depth;           /* depth of the deepest leaf (root = 0) */
+  uint64_t total_nodes;         /* sum of num_nodes over treelets       */
+  uint64_t tree_nodes;          /* distinct octree nodes                */
+  uint64_t tree_leaves;         /* octree leaves at max_depth           */
+} svo_scene_info;
+
+/* --------------------------------------------------------- scene handles */
+
+/* Reads <base_path>.svo and <base_path>.ia (TreeletMemoryManager::
+ * loadFromFile, src/TreeletMemoryManager.cpp:229-291).  The .ia layout is
+ * this library's own (u32 count, then per treelet u32 packages, u32 words
+ * per package = 2, then packages x {u32 packed_normal, u32 packed_color}).*/
+svo_status svo_scene_open(const char* base_path, svo_scene** out);
+
+/* Writes <base_path>.svo and <base_path>.ia
+ * (TreeletBuildManager::writeToFile, src/TreeletBuildManager.cpp:286-342).*/
+svo_status svo_scene_write(const svo_scene* scene, const char* base_path);
+
+void svo_scene_destroy(svo_scene* scene);
+
+svo_status svo_scene_get_info(const svo_scene* scene, svo_scene_info* out);
+
+/* Borrowed view of one treelet: header, num_nodes nodes and num_nodes
+ * attribute packages {packed_normal, packed_color}.  Valid until the scene
+ * is destroyed. */
+svo_status svo_scene_get_treelet(const svo_scene* scene, uint32_t gid,
+                                 svo_treelet_header* header,
+                                 const svo_node** nodes,
+                                 const uint32_t** attribs);
+
+/* Treelet packing of the scene builders (flags argument).
+ *   0                  dense packing (default): the octree is cut into the
+ *                      fewest connected subtrees of at most nodes_per_treelet
+ *                      nodes (greedy bottom-up, heaviest child cut first);
+ *                      each is stored breadth-first.
+ *   SVO_BUILD_PACK_BFS the reference builder's rule: fill each treelet
+ *                      breadth-first until fewer than 9 slots remain and start
+ *                      a sub-treelet at every unexpanded node above the scene
+ *                      depth (src/TreeletBuilderFromFaces.cpp:188-365,
+ *                      src/TreeletBuildManager.cpp:101-274).
+ * Both emit the reference's .svo invariants: root at local index 0,
+ * contiguous children with a relative first-child offset, sub-treelet anchor
+ * leaves holding the sub-treelet gid, gids in FIFO order over treelets,
+ * inner attributes = mean of the children. */
+enum { SVO_BUILD_PACK_BFS = 1u };
+
+/* Builds a treelet scene from leaf voxels at depth `depth`: `morton` holds
+ * strictly increasing Morton codes (3 bits per level, x most significant,
+ * i.e. child index 4x+2y+z), `attribs` two words per leaf
+ * {packed_normal, packed_color}.  treelet_bytes == 0 sizes one treelet to
+ * hold the whole tree. */
+svo_status svo_scene_build_from_leaves(uint32_t depth, const uint64_t* morton,
+                                       const uint32_t* attribs, size_t count,
+                                       uint32_t treelet_bytes, uint32_t flags,
+                                       svo_scene** out);
+
+/* Procedural scenes in the unit cube [-0.5,0.5]^3, stored as the surface
+ * shell of the solid (filled voxels with a face on empty space or on the
+ * cube boundary).
+ *   menger : level-`level` Menger sponge on a 2^depth grid, leaf colour =
+ *            hash(seed, i, j, k);
+ *   terrain: voxels whose centre lies below an 8-octave value-noise
+ *            heightfield h(x,z) in [0.2,0.8), colour by height band plus
+ *            hash noise;
+ *   spheres: `count` solid spheres, radius uniform in [r_min, r_max],
+ *            centres uniform in the cube. */
+svo_status svo_scene_build_menger(uint32_t level, uint32_t depth,
+                                  uint64_t seed, uint32_t treelet_bytes,
+                                  uint32_t flags, svo_scene** out);
+svo_status svo_scene_build_terrain(uint32_t depth, uint64_t seed,
+                                   uint32_t treelet_bytes, uint32_t flags,
+                                   svo_scene** out);
+svo_status svo_scene_build_spheres(uint32_t depth, uint32_t count,
+                                   float r_min, float r_max, uint64_t seed,
+                                   uint32_t treelet_bytes, uint32_t flags,
+                                   svo_scene** out);
+
+/* Height of the terrain surface h(x,z) in [0.2,0.8] (unit-cube units,
+ * y measured from the cube's bottom face) at x,z in [0,1]. */
+svo_status svo_terrain_height(uint64_t seed, double x, double z, double* h);
+
+/* ----------------------------------------------------------- camera */
+
+/* Per-frame ray-generation uniforms (renderToBuffer args 3-7,
+ * raycast_serialized_opencl.cpp:670-675,729-733), all FP32:
+ *   ray(x,y) dir = normalize(far_lower_left + pixel_h*x
+ *                            + pixel_v*(height - y) - position). */
+typedef struct svo_camera {
+  float far_lower_left[4];
+  float pixel_h[4];
+  float pixel_v[4];
+  float position[4];
+  float t_scale_ratio; /* LOD: frameBufferSize.z; 0 disables the LOD stop */
+  uint32_t width;
+  uint32_t height;
+  uint32_t reserved;
+} svo_camera;
+
+/* Pinhole camera at `eye` looking at `target` with `up`, vertical field of
+ * view fov_y_deg, far plane `far_dist`; fills the FP32 uniforms (computed in
+ * double, rounded once).  t_scale_ratio = lod_multiplier * tan(fov/2) *
+ * aspect / (width/2) (raycast_serialized_opencl.cpp:303-304). */
+svo_status svo_camera_look_at(const double eye[3], const double target[3],
+                              const double up[3], double fov_y_deg,
+                              uint32_t width, uint32_t height,
+                              double far_dist, double lod_multiplier,
+                              svo_camera* out);
+
+/* ------------------------------------------------ residency (HBM pools) */
+
+typedef struct svo_mm svo_mm;
+
+typedef struct svo_mm_info {
+  uint32_t num_slots;
+  uint32_t nodes_per_treelet;
+  uint32_t free_slots;
+  uint32_t dirty_slots;       /* slots waiting for svo_mm_update_device  */
+  uint32_t resident_treelets;
+  uint32_t device;
+  uint64_t pool_nodes;        /* num_slots * nodes_per_treelet           */
+  uint64_t device_bytes;      /* HBM held by the handle                  */
+} svo_mm_info;
+
+typedef struct svo_upload_stats {
+  uint32_t slots_uploaded;    /* zz_memory_manager-cl_slots_to_upload   */
+  uint32_t copy_calls;        /* zz_memory_manager-num_copy_calls       */
+  uint64_t bytes;
+} svo_upload_stats;
+
+/* TreeletMemoryManagerCl(path, incoreBufferSizeInByte, ctx) +
+ * initClBuffer(): slots = floor(incore_bytes / treelet_bytes), the root
+ * treelet inserted into slot 0 (resetIncoreBuffer,
+ * src/TreeletMemoryManager.cpp:303-380), HBM pools allocated on `device`
+ * and the whole incore buffer uploaded.  device = -1 creates a host-only
+ * handle (no HBM pools: residency bookkeeping only, svo_render refuses it).
+ * The scene must outlive the handle. */
+svo_status svo_mm_create(const svo_scene* scene, uint64_t incore_bytes,
+                         int device, svo_mm** out);
+void svo_mm_destroy(svo_mm* mm);
+
+/* resetIncoreBuffer(): empties every slot and re-inserts the root. */
+svo_status svo_mm_reset(svo_mm* mm);
+
+/* insertTreeletIntoIncoreBuffer(SlotInfo(gid, MAX_VISIBILITY, error))
+ * (src/TreeletMemoryManager.cpp:455-548), incl. eviction through
+ * freeIncorePosition when no slot is free (:560-653). */
+svo_status svo_mm_insert(svo_mm* mm, uint32_t gid, float error);
+
+/* removeTreeletFromIncoreBuffer(gid) (src/TreeletMemoryManager.cpp:665-700)*/
+svo_status svo_mm_remove(svo_mm* mm, uint32_t gid);
+
+/* Inserts every treelet in gid order (parents precede children). */
+svo_status svo_mm_make_resident_all(svo_mm* mm);
+
+/* updateDeviceMemory(): dirty slots merged into ranges with the reference
+ * ratio rule (src/TreeletMemoryManagerCl.cpp:155-251) and copied to HBM
+ * with hipMemcpyAsync on `stream`, followed by a stream synchronize. */
+svo_status svo_mm_update_device(svo_mm* mm, void* stream,
+                                svo_upload_stats* stats);
+
+svo_status svo_mm_get_info(const svo_mm* mm, svo_mm_info* out);
+
+/* gid held by `slot`, -1 if empty (getSlots()[slot]._treeletGid). */
+svo_status svo_mm_slot_treelet(const svo_mm* mm, uint32_t slot, int32_t* gid);
+
+/* slot holding `gid`, -1 if not resident (Treelet::getSlotGid). */
+svo_status svo_mm_treelet_slot(const svo_mm* mm, uint32_t gid, int32_t* slot);
+
+/* Copies nodes [first, first+count) of the HOST incore buffer
+ * (getIncoreBuffer) and their attribute packages (2 words each). */
+svo_status svo_mm_read_host_incore(const svo_mm* mm, uint64_t first,
+                                   uint64_t count, svo_node* nodes,
+                                   uint32_t* attribs);
+
+/* Same range read back from HBM (packed_normal/packed_color arrays). */
+svo_status svo_mm_read_device_incore(const svo_mm* mm, uint64_t first,
+                                     uint64_t count, svo_node* nodes,
+                                     uint32_t* attribs);
+
+/* ------------------------------------------------------------ rendering */
+
+/* Traversal policy.  SVO_POLICY_TREELET is the OpenCL renderer's
+ * sample() (opencl/treeletRenderer_rgba.cl:201-382); SVO_POLICY_CPU3 is
+ * CpuRaycasterSingleRay3::traversSvo
+ * (src/incore/CpuRaycasterSingleRay3.cpp:158-387) run on the same
+ * relative-indexed incore buffer. */
+enum { SVO_POLICY_TREELET = 0, SVO_POLICY_CPU3 = 2 };
+
+/* Per-camera outputs, device pointers, width*height elements each, row y
+ * at offset y*width (y as in renderToBuffer's get_global_id(1)).  Any
+ * pointer may be NULL to skip that output.
+ *   t       : FP32 ray parameter of the hit (parentTMin), the box exit
+ *             tMax on a miss inside the box, +inf if the box is missed
+ *   node    : incore node index of the hit, 0xFFFFFFFF on a miss
+ *   treelet : gid of the treelet whose slot holds `node`, 0xFFFFFFFF miss
+ *   rgba    : packed_color of the hit node with alpha forced to 255
+ *             (r<<24|g<<16|b<<8|a); miss = background 0.2,0.3,0.2,1
+ *             -> 0x334D33FF
+ *   pushes  : number of stack pushes of the ray (traversal statistics;
+ *             8 bytes of node data are fetched per push)                */
+typedef struct svo_frame_out {
+  float* t;
+  uint32_t* node;
+  uint32_t* treelet;
+  uint32_t* rgba;
+  uint32_t* pushes;
+} svo_frame_out;
+
+/* Renders n_cams frames in one launch on `stream` (asynchronous). */
+svo_status svo_render(svo_mm* mm, const svo_camera* cams, uint32_t n_cams,
+                      const svo_frame_out* outs, int policy, void* stream);
+
+/* ------------------------------------------- device memory and streams */
+
+svo_status svo_device_count(int* count);
+svo_status svo_set_device(int device);
+svo_status svo_device_malloc(void** ptr, size_t bytes);
+svo_status svo_device_free(void* ptr);
+svo_status svo_memcpy_h2d(void* dst, const void* src, size_t bytes);
+svo_status svo_memcpy_d2h(void* dst, const void* src, size_t bytes);
+svo_status svo_memset_device(void* dst, int value, size_t bytes);
+svo_status svo_stream_create(void** stream);
+svo_status svo_stream_destroy(void* stream);
+svo_status svo_stream_synchronize(void* stream);
+svo_status svo_device_synchronize(void);
+
+/* Times `iters` back-to-back svo_render launches on `stream` with HIP
+ * events recorded on that stream; returns the mean ms per launch. */
+svo_status svo_render_timed(svo_mm* mm, const svo_camera* cams,
+                            uint32_t n_cams, const svo_frame_out* outs,
+                            int policy, void* stream, uint32_t iters,
+                            float* mean_ms);
+
+/* Diagnostic: writes the FP32 ray direction of every pixel of *cam
+ * (3 floats per pixel, row-major) into dirs_dev, computed exactly as the
+ * traversal kernel computes it (renderToBuffer's ray generation,
+ * opencl/treeletRenderer_rgba.cl:627-633). */
+svo_status svo_debug_ray_dirs(const svo_camera* cam, float* dirs_dev,
+                              void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SVO_HIP_H */

@@ -1,0 +1,341 @@
+"""MI355X serialized-SVO treelet ray caster: host-side Python mirror.
+
+Everything here is a thin layer over the C-ABI of ``libsvo_hip.so``
+(include/svo_hip.h); the traversal runs only in the HIP kernels.  Class and
+method names follow the reference's C++ API so that code written against
+libsvo reads the same:
+
+* :class:`Scene`                   -- the treelet bundle (``.svo`` + ``.ia``)
+  as loaded by ``TreeletMemoryManager::loadFromFile``
+  (src/TreeletMemoryManager.cpp:229-291) or built procedurally.
+* :class:`TreeletMemoryManagerHip` -- ``TreeletMemoryManagerCl`` with HBM
+  pools (src/TreeletMemoryManager.h:97-157, src/TreeletMemoryManagerCl.h:69-93).
+* :func:`render_to_buffer`         -- the ``renderToBuffer`` launch
+  (opencl/treeletRenderer_rgba.cl:610-655) for one or many cameras.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+
+from . import _capi
+from ._capi import (BACKGROUND_RGBA, MISS, SVO_BUILD_PACK_BFS, SVO_POLICY_CPU3, SVO_POLICY_TREELET, SvoError,
+                    check, lib, svo_camera, svo_frame_out)
+
+__all__ = [
+    "Scene", "TreeletMemoryManagerHip", "Camera", "DeviceBuffer", "Stream", "Frame", "render_to_buffer",
+    "device_count", "look_at", "SvoError", "POLICIES", "MISS", "BACKGROUND_RGBA",
+]
+
+POLICIES = {"treelet": SVO_POLICY_TREELET, "cpu3": SVO_POLICY_CPU3}
+Camera = svo_camera
+
+
+def _policy(p) -> int:
+    return POLICIES[p] if isinstance(p, str) else int(p)
+
+
+def _packing(packing: str) -> int:
+    if packing not in ("dense", "bfs"):
+        raise ValueError(f"packing must be 'dense' or 'bfs', not {packing!r}")
+    return SVO_BUILD_PACK_BFS if packing == "bfs" else 0
+
+
+def device_count() -> int:
+    n = ctypes.c_int(0)
+    if lib.svo_device_count(ctypes.byref(n)) != 0:
+        return 0
+    return n.value
+
+
+def look_at(eye, target, up=(0.0, 1.0, 0.0), fov_y_deg: float = 60.0, width: int = 512, height: int = 512,
+            far: float = 20.0, lod: float = 0.0) -> svo_camera:
+    """Pinhole camera uniforms; ``lod`` is the reference's tScaleRatio
+    multiplier (0 disables the level-of-detail stop)."""
+    c = svo_camera()
+    arr = lambda v: (ctypes.c_double * 3)(*[float(x) for x in v])
+    check(lib.svo_camera_look_at(arr(eye), arr(target), arr(up), float(fov_y_deg), int(width), int(height),
+                                 float(far), float(lod), ctypes.byref(c)), "svo_camera_look_at")
+    return c
+
+
+# ------------------------------------------------------------------ scenes
+
+class Scene:
+    """A treelet bundle held in host memory."""
+
+    def __init__(self, handle: ctypes.c_void_p):
+        self._h = handle
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib.svo_scene_destroy(h)
+            self._h = None
+
+    @property
+    def handle(self) -> ctypes.c_void_p:
+        return self._h
+
+    @classmethod
+    def _build(cls, fn, *args) -> "Scene":
+        h = ctypes.c_void_p()
+        check(fn(*args, ctypes.byref(h)), fn.__name__)
+        return cls(h)
+
+    @classmethod
+    def open(cls, base_path: str) -> "Scene":
+        return cls._build(lib.svo_scene_open, base_path.encode())
+
+    @classmethod
+    def menger(cls, level: int, depth: int, seed: int, treelet_bytes: int = 0, packing: str = "dense") -> "Scene":
+        return cls._build(lib.svo_scene_build_menger, level, depth, seed, treelet_bytes, _packing(packing))
+
+    @classmethod
+    def terrain(cls, depth: int, seed: int, treelet_bytes: int = 32768, packing: str = "dense") -> "Scene":
+        return cls._build(lib.svo_scene_build_terrain, depth, seed, treelet_bytes, _packing(packing))
+
+    @classmethod
+    def spheres(cls, depth: int, count: int, r_min: float, r_max: float, seed: int, treelet_bytes: int = 32768,
+                packing: str = "dense") -> "Scene":
+        return cls._build(lib.svo_scene_build_spheres, depth, count, r_min, r_max, seed, treelet_bytes,
+                          _packing(packing))
+
+    @classmethod
+    def from_leaves(cls, depth: int, morton: np.ndarray, attribs: np.ndarray, treelet_bytes: int = 0,
+                    packing: str = "dense") -> "Scene":
+        m = np.ascontiguousarray(morton, dtype=np.uint64)
+        a = np.ascontiguousarray(attribs, dtype=np.uint32).reshape(-1)
+        if a.size != 2 * m.size:
+            raise ValueError("attribs must hold two words per leaf")
+        return cls._build(lib.svo_scene_build_from_leaves, depth, m.ctypes.data, a.ctypes.data, m.size,
+                          treelet_bytes, _packing(packing))
+
+    def write(self, base_path: str) -> None:
+        check(lib.svo_scene_write(self._h, base_path.encode()), "svo_scene_write")
+
+    @property
+    def info(self) -> Dict[str, int]:
+        i = _capi.svo_scene_info()
+        check(lib.svo_scene_get_info(self._h, ctypes.byref(i)), "svo_scene_get_info")
+        return {k: getattr(i, k) for k, _ in _capi.svo_scene_info._fields_}
+
+    def treelet(self, gid: int):
+        """(header dict, nodes (n,2) uint32 copy, attribs (n,2) uint32 copy)."""
+        h = _capi.svo_treelet_header()
+        pn, pa = ctypes.c_void_p(), ctypes.c_void_p()
+        check(lib.svo_scene_get_treelet(self._h, gid, ctypes.byref(h), ctypes.byref(pn), ctypes.byref(pa)),
+              "svo_scene_get_treelet")
+        n = h.num_nodes
+        nodes = np.ctypeslib.as_array(ctypes.cast(pn, ctypes.POINTER(ctypes.c_uint32)), (n, 2)).copy()
+        attr = np.ctypeslib.as_array(ctypes.cast(pa, ctypes.POINTER(ctypes.c_uint32)), (n, 2)).copy()
+        return {k: getattr(h, k) for k, _ in _capi.svo_treelet_header._fields_}, nodes, attr
+
+    def getTreeletSizeInByte(self) -> int:
+        return self.info["treelet_bytes"]
+
+    def getNumNodesPerTreelet(self) -> int:
+        return self.info["nodes_per_treelet"]
+
+
+# ------------------------------------------------------------ device memory
+
+class DeviceBuffer:
+    """Raw HBM allocation owned by Python."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        self.ptr = ctypes.c_void_p()
+        check(lib.svo_device_malloc(ctypes.byref(self.ptr), max(1, self.nbytes)), "svo_device_malloc")
+
+    def __del__(self):
+        if getattr(self, "ptr", None) and self.ptr.value:
+            lib.svo_device_free(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def to_numpy(self, dtype, count: Optional[int] = None) -> np.ndarray:
+        dt = np.dtype(dtype)
+        n = self.nbytes // dt.itemsize if count is None else int(count)
+        out = np.empty(n, dtype=dt)
+        check(lib.svo_memcpy_d2h(out.ctypes.data, self.ptr, n * dt.itemsize), "svo_memcpy_d2h")
+        return out
+
+    def fill(self, byte: int) -> None:
+        check(lib.svo_memset_device(self.ptr, byte, self.nbytes), "svo_memset_device")
+
+
+class Stream:
+    def __init__(self):
+        self.handle = ctypes.c_void_p()
+        check(lib.svo_stream_create(ctypes.byref(self.handle)), "svo_stream_create")
+
+    def synchronize(self) -> None:
+        check(lib.svo_stream_synchronize(self.handle), "svo_stream_synchronize")
+
+    def __del__(self):
+        if getattr(self, "handle", None) and self.handle.value:
+            lib.svo_stream_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
+
+
+# ------------------------------------------------------------ residency
+
+class TreeletMemoryManagerHip:
+    """Incore buffer of ``incore_bytes`` split into treelet slots, mirrored in
+    HBM.  Construction inserts the root treelet and uploads the whole buffer
+    (``TreeletMemoryManagerCl`` ctor + ``initClBuffer``)."""
+
+    def __init__(self, scene: Scene, incore_bytes: int, device: int = 0, resident_all: bool = False):
+        self.scene = scene  # keeps the host bundle alive
+        self._h = ctypes.c_void_p()
+        check(lib.svo_mm_create(scene.handle, int(incore_bytes), int(device), ctypes.byref(self._h)),
+              "svo_mm_create")
+        if resident_all:
+            self.make_resident_all()
+            self.updateDeviceMemory()
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib.svo_mm_destroy(h)
+            self._h = None
+
+    @property
+    def handle(self) -> ctypes.c_void_p:
+        return self._h
+
+    @property
+    def info(self) -> Dict[str, int]:
+        i = _capi.svo_mm_info()
+        check(lib.svo_mm_get_info(self._h, ctypes.byref(i)), "svo_mm_get_info")
+        return {k: getattr(i, k) for k, _ in _capi.svo_mm_info._fields_}
+
+    def resetIncoreBuffer(self) -> None:
+        check(lib.svo_mm_reset(self._h), "svo_mm_reset")
+
+    def insertTreeletIntoIncoreBuffer(self, gid: int, error: float = 0.0) -> None:
+        check(lib.svo_mm_insert(self._h, gid, error), "svo_mm_insert")
+
+    def removeTreeletFromIncoreBuffer(self, gid: int) -> None:
+        check(lib.svo_mm_remove(self._h, gid), "svo_mm_remove")
+
+    def make_resident_all(self) -> None:
+        check(lib.svo_mm_make_resident_all(self._h), "svo_mm_make_resident_all")
+
+    def updateDeviceMemory(self, stream: Optional[Stream] = None) -> Dict[str, int]:
+        s = _capi.svo_upload_stats()
+        check(lib.svo_mm_update_device(self._h, stream.handle if stream else None, ctypes.byref(s)),
+              "svo_mm_update_device")
+        return {k: getattr(s, k) for k, _ in _capi.svo_upload_stats._fields_}
+
+    def getNumNodesPerTreelet(self) -> int:
+        return self.info["nodes_per_treelet"]
+
+    def getSlots(self) -> List[int]:
+        out = []
+        g = ctypes.c_int32()
+        for s in range(self.info["num_slots"]):
+            check(lib.svo_mm_slot_treelet(self._h, s, ctypes.byref(g)), "svo_mm_slot_treelet")
+            out.append(g.value)
+        return out
+
+    def getSlotGid(self, gid: int) -> int:
+        s = ctypes.c_int32()
+        check(lib.svo_mm_treelet_slot(self._h, gid, ctypes.byref(s)), "svo_mm_treelet_slot")
+        return s.value
+
+    def getIncoreBuffer(self, first: int = 0, count: Optional[int] = None):
+        """(nodes (n,2) uint32, attribs (n,2) uint32) of the host incore buffer."""
+        n = self.info["pool_nodes"] - first if count is None else count
+        nodes = np.empty((n, 2), np.uint32)
+        attr = np.empty((n, 2), np.uint32)
+        check(lib.svo_mm_read_host_incore(self._h, first, n, nodes.ctypes.data, attr.ctypes.data),
+              "svo_mm_read_host_incore")
+        return nodes, attr
+
+    def getDeviceIncoreBuffer(self, first: int = 0, count: Optional[int] = None):
+        n = self.info["pool_nodes"] - first if count is None else count
+        nodes = np.empty((n, 2), np.uint32)
+        attr = np.empty((n, 2), np.uint32)
+        check(lib.svo_mm_read_device_incore(self._h, first, n, nodes.ctypes.data, attr.ctypes.data),
+              "svo_mm_read_device_incore")
+        return nodes, attr
+
+
+# ------------------------------------------------------------ rendering
+
+@dataclass
+class Frame:
+    """Device outputs of one camera (width*height each)."""
+    width: int
+    height: int
+    t: DeviceBuffer
+    node: DeviceBuffer
+    treelet: DeviceBuffer
+    rgba: DeviceBuffer
+    pushes: Optional[DeviceBuffer] = None
+
+    @classmethod
+    def alloc(cls, width: int, height: int, pushes: bool = False) -> "Frame":
+        n = width * height
+        return cls(width, height, DeviceBuffer(4 * n), DeviceBuffer(4 * n), DeviceBuffer(4 * n), DeviceBuffer(4 * n),
+                   DeviceBuffer(4 * n) if pushes else None)
+
+    def out(self) -> svo_frame_out:
+        return svo_frame_out(self.t.ptr, self.node.ptr, self.treelet.ptr, self.rgba.ptr,
+                             self.pushes.ptr if self.pushes is not None else None)
+
+    def fetch(self) -> Dict[str, np.ndarray]:
+        shp = (self.height, self.width)
+        out = {"t": self.t.to_numpy(np.float32).reshape(shp), "node": self.node.to_numpy(np.uint32).reshape(shp),
+               "treelet": self.treelet.to_numpy(np.uint32).reshape(shp),
+               "rgba": self.rgba.to_numpy(np.uint32).reshape(shp)}
+        if self.pushes is not None:
+            out["pushes"] = self.pushes.to_numpy(np.uint32).reshape(shp)
+        return out
+
+
+def _cam_array(cams: Sequence[svo_camera]):
+    arr = (svo_camera * len(cams))()
+    for i, c in enumerate(cams):
+        arr[i] = c
+    return arr
+
+
+def render_to_buffer(mm: TreeletMemoryManagerHip, cams: Sequence[svo_camera], frames: Sequence[Frame],
+                     policy="treelet", stream: Optional[Stream] = None) -> None:
+    """Enqueues one launch that renders every camera into its frame."""
+    if len(cams) != len(frames) or not cams:
+        raise ValueError("one frame per camera required")
+    outs = (svo_frame_out * len(frames))(*[f.out() for f in frames])
+    check(lib.svo_render(mm.handle, _cam_array(cams), len(cams), outs, _policy(policy),
+                         stream.handle if stream else None), "svo_render")
+
+
+def render_timed(mm: TreeletMemoryManagerHip, cams: Sequence[svo_camera], frames: Sequence[Frame], policy="treelet",
+                 stream: Optional[Stream] = None, iters: int = 10) -> float:
+    """Mean milliseconds per launch, timed with HIP events on the launch stream."""
+    outs = (svo_frame_out * len(frames))(*[f.out() for f in frames])
+    ms = ctypes.c_float()
+    check(lib.svo_render_timed(mm.handle, _cam_array(cams), len(cams), outs, _policy(policy),
+                               stream.handle if stream else None, iters, ctypes.byref(ms)), "svo_render_timed")
+    return ms.value
+
+
+def render(mm: TreeletMemoryManagerHip, cam: svo_camera, policy="treelet", pushes: bool = True) -> Dict[str, np.ndarray]:
+    """Renders one camera and returns host copies of t, node, treelet, rgba
+    (and the per-ray push counts)."""
+    f = Frame.alloc(cam.width, cam.height, pushes)
+    render_to_buffer(mm, [cam], [f], policy)
+    check(lib.svo_device_synchronize(), "svo_device_synchronize")
+    return f.fetch()
+
+
+def ray_dirs(cam: svo_camera) -> np.ndarray:
+    buf = DeviceBuffer(12 * cam.width * cam.height)
+    check(lib.svo_debug_ray_dirs(ctypes.byref(cam), buf.ptr, None), "svo_debug_ray_dirs")
+    check(lib.svo_device_synchronize(), "svo_device_synchronize")
+    return buf.to_numpy(np.float32).reshape(cam.height, cam.width, 3)

@@ -1,0 +1,127 @@
+"""ctypes bindings of libsvo_hip.so (include/svo_hip.h).
+
+The library is built in-tree by ``make -C libsvo_amd`` (``__graft_entry__.build``).
+Importing this module without the built library raises ImportError: there is
+no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsvo_hip.so")
+
+u32, u64, i32, f32, f64 = ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int32, ctypes.c_float, ctypes.c_double
+vp = ctypes.c_void_p
+P = ctypes.POINTER
+
+SVO_OK = 0
+SVO_ERR_ARG, SVO_ERR_IO, SVO_ERR_FORMAT, SVO_ERR_HIP, SVO_ERR_NOMEM, SVO_ERR_STATE, SVO_ERR_CAPACITY = range(1, 8)
+SVO_POLICY_TREELET = 0
+SVO_POLICY_CPU3 = 2
+SVO_BUILD_PACK_BFS = 1
+MISS = 0xFFFFFFFF
+BACKGROUND_RGBA = 0x334D33FF
+
+
+class svo_node(ctypes.Structure):
+    _fields_ = [("first_child", u32), ("masks", u32)]
+
+
+class svo_treelet_header(ctypes.Structure):
+    _fields_ = [(n, u32) for n in ("num_nodes", "num_leaves", "mem_size", "gid", "parent_gid",
+                                    "parent_leaf_pos", "parent_leaf_idx", "parent_leaf_parent_pos")]
+
+
+class svo_scene_info(ctypes.Structure):
+    _fields_ = [("num_treelets", u32), ("treelet_bytes", u32), ("nodes_per_treelet", u32),
+                ("max_depth", u32), ("total_nodes", u64), ("tree_nodes", u64), ("tree_leaves", u64)]
+
+
+class svo_camera(ctypes.Structure):
+    _fields_ = [("far_lower_left", f32 * 4), ("pixel_h", f32 * 4), ("pixel_v", f32 * 4),
+                ("position", f32 * 4), ("t_scale_ratio", f32), ("width", u32), ("height", u32),
+                ("reserved", u32)]
+
+
+class svo_mm_info(ctypes.Structure):
+    _fields_ = [("num_slots", u32), ("nodes_per_treelet", u32), ("free_slots", u32),
+                ("dirty_slots", u32), ("resident_treelets", u32), ("device", u32),
+                ("pool_nodes", u64), ("device_bytes", u64)]
+
+
+class svo_upload_stats(ctypes.Structure):
+    _fields_ = [("slots_uploaded", u32), ("copy_calls", u32), ("bytes", u64)]
+
+
+class svo_frame_out(ctypes.Structure):
+    _fields_ = [("t", vp), ("node", vp), ("treelet", vp), ("rgba", vp), ("pushes", vp)]
+
+
+# name -> (restype, argtypes); every function declared in include/svo_hip.h
+SIGNATURES = {
+    "svo_last_error": (ctypes.c_char_p, []),
+    "svo_scene_open": (ctypes.c_int, [ctypes.c_char_p, P(vp)]),
+    "svo_scene_write": (ctypes.c_int, [vp, ctypes.c_char_p]),
+    "svo_scene_destroy": (None, [vp]),
+    "svo_scene_get_info": (ctypes.c_int, [vp, P(svo_scene_info)]),
+    "svo_scene_get_treelet": (ctypes.c_int, [vp, u32, P(svo_treelet_header), P(vp), P(vp)]),
+    "svo_scene_build_from_leaves": (ctypes.c_int, [u32, vp, vp, ctypes.c_size_t, u32, u32, P(vp)]),
+    "svo_scene_build_menger": (ctypes.c_int, [u32, u32, u64, u32, u32, P(vp)]),
+    "svo_scene_build_terrain": (ctypes.c_int, [u32, u64, u32, u32, P(vp)]),
+    "svo_scene_build_spheres": (ctypes.c_int, [u32, u32, f32, f32, u64, u32, u32, P(vp)]),
+    "svo_terrain_height": (ctypes.c_int, [u64, f64, f64, P(f64)]),
+    "svo_camera_look_at": (ctypes.c_int, [P(f64), P(f64), P(f64), f64, u32, u32, f64, f64, P(svo_camera)]),
+    "svo_mm_create": (ctypes.c_int, [vp, u64, ctypes.c_int, P(vp)]),
+    "svo_mm_destroy": (None, [vp]),
+    "svo_mm_reset": (ctypes.c_int, [vp]),
+    "svo_mm_insert": (ctypes.c_int, [vp, u32, f32]),
+    "svo_mm_remove": (ctypes.c_int, [vp, u32]),
+    "svo_mm_make_resident_all": (ctypes.c_int, [vp]),
+    "svo_mm_update_device": (ctypes.c_int, [vp, vp, P(svo_upload_stats)]),
+    "svo_mm_get_info": (ctypes.c_int, [vp, P(svo_mm_info)]),
+    "svo_mm_slot_treelet": (ctypes.c_int, [vp, u32, P(i32)]),
+    "svo_mm_treelet_slot": (ctypes.c_int, [vp, u32, P(i32)]),
+    "svo_mm_read_host_incore": (ctypes.c_int, [vp, u64, u64, vp, vp]),
+    "svo_mm_read_device_incore": (ctypes.c_int, [vp, u64, u64, vp, vp]),
+    "svo_render": (ctypes.c_int, [vp, P(svo_camera), u32, P(svo_frame_out), ctypes.c_int, vp]),
+    "svo_device_count": (ctypes.c_int, [P(ctypes.c_int)]),
+    "svo_set_device": (ctypes.c_int, [ctypes.c_int]),
+    "svo_device_malloc": (ctypes.c_int, [P(vp), ctypes.c_size_t]),
+    "svo_device_free": (ctypes.c_int, [vp]),
+    "svo_memcpy_h2d": (ctypes.c_int, [vp, vp, ctypes.c_size_t]),
+    "svo_memcpy_d2h": (ctypes.c_int, [vp, vp, ctypes.c_size_t]),
+    "svo_memset_device": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_size_t]),
+    "svo_stream_create": (ctypes.c_int, [P(vp)]),
+    "svo_stream_destroy": (ctypes.c_int, [vp]),
+    "svo_stream_synchronize": (ctypes.c_int, [vp]),
+    "svo_device_synchronize": (ctypes.c_int, []),
+    "svo_render_timed": (ctypes.c_int, [vp, P(svo_camera), u32, P(svo_frame_out), ctypes.c_int, vp, u32, P(f32)]),
+    "svo_debug_ray_dirs": (ctypes.c_int, [P(svo_camera), vp, vp]),
+}
+
+
+class SvoError(RuntimeError):
+    def __init__(self, status: int, where: str, msg: str):
+        super().__init__(f"{where}: status {status}: {msg}")
+        self.status = status
+
+
+def load(path: str = LIB_PATH) -> ctypes.CDLL:
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is not built; run __graft_entry__.build() (make -C libsvo_amd)")
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = load()
+
+
+def check(status: int, where: str) -> None:
+    if status != SVO_OK:
+        raise SvoError(status, where, lib.svo_last_error().decode(errors="replace"))

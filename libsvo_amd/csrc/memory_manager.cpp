@@ -1,0 +1,576 @@
+// Treelet residency: the host incore buffer with the reference's slot, stitch
+// and evict semantics (src/TreeletMemoryManager.cpp), backed by HBM pools
+// that replace the OpenCL buffers of TreeletMemoryManagerCl
+// (src/TreeletMemoryManagerCl.cpp).
+//
+// HBM layout of one handle (all pools indexed by incore node index
+// n = slot * nodes_per_treelet + local):
+//   nodes    : 8 B per node {first_child, masks}     (device copy of
+//              getIncoreBuffer())
+//   colors   : 4 B per node packed_color             (the half of the
+//   normals  : 4 B per node packed_normal             reference's 8 B Attribs
+//              that the colour epilogue reads is a separate, dense array)
+//   slot_gid : 4 B per slot, gid held by the slot or -1
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <queue>
+#include <set>
+#include <vector>
+
+#include "svo_internal.h"
+#include "trace.h"
+
+using namespace svo_impl;
+
+namespace {
+
+constexpr int kMaxVisibility = 8;  // MAX_VISIBILITY, src/TreeletMemoryManager.cpp:47
+
+struct SlotInfo {  // TreeletMemoryManager::SlotInfo (src/TreeletMemoryManager.h:72-89)
+  int32_t gid = -1;
+  int visibility = 0;
+  float error = 100000.0f;
+};
+
+#define HIP_TRY(expr)                                                                   \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess) {                                                             \
+      set_error("%s failed: %s", #expr, hipGetErrorString(e_));                         \
+      return SVO_ERR_HIP;                                                               \
+    }                                                                                   \
+  } while (0)
+
+}  // namespace
+
+struct svo_mm {
+  const svo_scene* scene = nullptr;
+  uint32_t N = 0;           // nodes per treelet
+  uint32_t num_slots = 0;
+  int device = 0;
+  // host incore buffer (SoA attributes)
+  std::vector<svo_node> incore;
+  std::vector<uint32_t> normals, colors;
+  std::vector<SlotInfo> slots;
+  std::vector<int32_t> treelet_slot;                 // Treelet::getSlotGid, -1 = not resident
+  std::vector<std::set<uint32_t>> children_resident;  // _childTreeletsInIncoreBuffer
+  std::vector<uint32_t> free_slots;                  // stack, top = back()
+  std::set<uint32_t> dirty;                          // _incoreSlotsToUpload
+  uint32_t search_pos = 1;                           // freeIncorePosition's probe cursor
+  uint32_t resident = 0;
+  // HBM
+  uint64_t* d_nodes = nullptr;
+  uint32_t* d_colors = nullptr;
+  uint32_t* d_normals = nullptr;
+  int32_t* d_slot_gid = nullptr;
+  TraceFrame* d_frames = nullptr;
+  uint32_t frames_cap = 0;
+  TraceFrame* h_frames = nullptr;  // pinned staging for frame descriptors
+  hipEvent_t frames_ready = nullptr;
+  bool frames_pending = false;
+
+  bool host_only() const { return device < 0; }
+
+  ~svo_mm() {
+    if (host_only()) return;
+    int cur = 0;
+    hipGetDevice(&cur);
+    hipSetDevice(device);
+    if (frames_pending) hipEventSynchronize(frames_ready);
+    if (frames_ready) hipEventDestroy(frames_ready);
+    hipFree(d_nodes);
+    hipFree(d_colors);
+    hipFree(d_normals);
+    hipFree(d_slot_gid);
+    hipFree(d_frames);
+    if (h_frames) hipHostFree(h_frames);
+    hipSetDevice(cur);
+  }
+
+  void mark(uint32_t slot) { dirty.insert(slot); }
+
+  // resetIncoreBuffer (src/TreeletMemoryManager.cpp:303-380), host part.
+  void reset_host() {
+    std::fill(incore.begin(), incore.end(), svo_node{0, 0});
+    std::fill(normals.begin(), normals.end(), 0u);
+    std::fill(colors.begin(), colors.end(), 0u);
+    slots.assign(num_slots, SlotInfo());
+    treelet_slot.assign(scene->treelets.size(), -1);
+    children_resident.assign(scene->treelets.size(), std::set<uint32_t>());
+    free_slots.clear();
+    for (uint32_t i = 1; i != num_slots + 1; ++i) free_slots.push_back(num_slots - i);  // top = slot 0
+    dirty.clear();
+    search_pos = 1;
+    resident = 0;
+  }
+
+  // removeTreeletFromIncoreBuffer (src/TreeletMemoryManager.cpp:665-700).
+  void remove_one(uint32_t gid) {
+    const svo_treelet_header& h = scene->treelets[gid].h;
+    const int32_t slot = treelet_slot[gid];
+    const int32_t pslot = treelet_slot[h.parent_gid];
+    const uint64_t leaf = (uint64_t)pslot * N + h.parent_leaf_pos;
+    const uint64_t leaf_parent = (uint64_t)pslot * N + h.parent_leaf_parent_pos;
+    incore[leaf] = scene->treelets[h.parent_gid].nodes[h.parent_leaf_pos];  // pristine leaf
+    incore[leaf_parent].masks |= 1u << (kLeafShift + h.parent_leaf_idx);
+    mark((uint32_t)pslot);
+    slots[slot] = SlotInfo();
+    free_slots.push_back((uint32_t)slot);
+    treelet_slot[gid] = -1;
+    children_resident[h.parent_gid].erase(gid);
+    --resident;
+  }
+
+  // Frees the resident sub-treelets below `gid` (the descendant sweep of
+  // freeIncorePosition, src/TreeletMemoryManager.cpp:603-648).
+  void free_descendants(uint32_t gid) {
+    std::queue<uint32_t> q;
+    q.push(gid);
+    std::vector<uint32_t> clear;
+    while (!q.empty()) {
+      const uint32_t p = q.front();
+      q.pop();
+      for (uint32_t c : children_resident[p]) { q.push(c); clear.push_back(c); }
+      children_resident[p].clear();
+    }
+    for (uint32_t c : clear) {
+      const int32_t s = treelet_slot[c];
+      if (s <= 0) continue;  // slot 0 (the root) is never freed
+      free_slots.push_back((uint32_t)s);
+      treelet_slot[c] = -1;
+      slots[s] = SlotInfo();
+      --resident;
+    }
+  }
+
+  // freeIncorePosition (src/TreeletMemoryManager.cpp:560-653): probes up to
+  // 100 slots from the cursor for an invisible treelet whose parent is
+  // invisible too, then frees it and everything resident below it.
+  bool free_position() {
+    uint32_t freeable = 0;
+    uint32_t tests = 0;
+    const uint32_t probes = 100;
+    for (uint32_t i = 0; i != num_slots; ++i) {
+      const SlotInfo& s = slots[search_pos];
+      if (s.visibility < 0 && s.gid != -1) {
+        const uint32_t parent = scene->treelets[s.gid].h.parent_gid;
+        const int32_t ps = treelet_slot[parent];
+        if (ps >= 0 && slots[ps].visibility < 0) { freeable = (uint32_t)s.gid; break; }
+      }
+      ++search_pos;
+      ++tests;
+      if (search_pos == num_slots) search_pos = 1;
+      if (tests == probes) break;
+    }
+    if (freeable == 0) return false;
+    free_descendants(freeable);
+    remove_one(freeable);
+    return true;
+  }
+
+  // insertTreeletIntoIncoreBuffer (src/TreeletMemoryManager.cpp:455-548).
+  svo_status insert(uint32_t gid, float error) {
+    if (gid >= scene->treelets.size()) { set_error("insert: treelet gid %u out of range", gid); return SVO_ERR_ARG; }
+    if (treelet_slot[gid] >= 0) { set_error("insert: treelet %u is already resident", gid); return SVO_ERR_STATE; }
+    const svo_treelet_header& h = scene->treelets[gid].h;
+    if (gid != 0 && treelet_slot[h.parent_gid] < 0) {
+      set_error("insert: parent treelet %u of %u is not resident", h.parent_gid, gid);
+      return SVO_ERR_STATE;
+    }
+    if (free_slots.empty() && !free_position()) { set_error("insert: no free or freeable incore slot for treelet %u", gid); return SVO_ERR_CAPACITY; }
+    // eviction may have removed the parent
+    if (gid != 0 && treelet_slot[h.parent_gid] < 0) {
+      set_error("insert: parent treelet %u of %u was evicted", h.parent_gid, gid);
+      return SVO_ERR_CAPACITY;
+    }
+    const uint32_t slot = free_slots.back();
+    free_slots.pop_back();
+    slots[slot] = SlotInfo{(int32_t)gid, kMaxVisibility, error};
+    treelet_slot[gid] = (int32_t)slot;
+    ++resident;
+    const uint64_t pos = (uint64_t)slot * N;
+    const TreeletData& T = scene->treelets[gid];
+    std::copy(T.nodes.begin(), T.nodes.end(), incore.begin() + pos);
+    std::fill(incore.begin() + pos + T.h.num_nodes, incore.begin() + pos + N, svo_node{0, 0});
+    for (uint32_t i = 0; i < T.h.num_nodes; ++i) {
+      normals[pos + i] = T.attribs[2 * (size_t)i];
+      colors[pos + i] = T.attribs[2 * (size_t)i + 1];
+    }
+    std::fill(normals.begin() + pos + T.h.num_nodes, normals.begin() + pos + N, 0u);
+    std::fill(colors.begin() + pos + T.h.num_nodes, colors.begin() + pos + N, 0u);
+    mark(slot);
+    if (gid == 0) return SVO_OK;
+    // stitch: the child root replaces the parent's leaf, which stops being a
+    // leaf and points (relatively, modulo 2^32) at the child's level 1.
+    const int32_t pslot = treelet_slot[h.parent_gid];
+    const uint64_t leaf = (uint64_t)pslot * N + h.parent_leaf_pos;
+    const uint64_t leaf_parent = (uint64_t)pslot * N + h.parent_leaf_parent_pos;
+    incore[leaf] = incore[pos];
+    incore[leaf_parent].masks &= ~(1u << (kLeafShift + h.parent_leaf_idx));
+    incore[leaf].first_child = (uint32_t)(pos + 1) - (uint32_t)leaf;
+    mark((uint32_t)pslot);
+    children_resident[h.parent_gid].insert(gid);
+    return SVO_OK;
+  }
+
+  svo_status upload_range(uint32_t first_slot, uint32_t count, hipStream_t st, uint64_t* bytes) {
+    const size_t n0 = (size_t)first_slot * N, n = (size_t)count * N;
+    if (bytes) *bytes += n * 16;
+    if (host_only()) return SVO_OK;
+    HIP_TRY(hipMemcpyAsync(d_nodes + n0, incore.data() + n0, n * 8, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(d_colors + n0, colors.data() + n0, n * 4, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(d_normals + n0, normals.data() + n0, n * 4, hipMemcpyHostToDevice, st));
+    return SVO_OK;
+  }
+
+  svo_status upload_slot_table(hipStream_t st) {
+    if (host_only()) return SVO_OK;
+    std::vector<int32_t> tab(num_slots);
+    for (uint32_t s = 0; s < num_slots; ++s) tab[s] = slots[s].gid;
+    HIP_TRY(hipMemcpyAsync(d_slot_gid, tab.data(), (size_t)num_slots * 4, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return SVO_OK;
+  }
+
+  // updateDeviceMemory (src/TreeletMemoryManagerCl.cpp:155-251): sorted
+  // dirty slots are merged while (treelets+1)/(distance+1) >= 0.6*2.
+  svo_status update_device(hipStream_t st, svo_upload_stats* stats) {
+    svo_upload_stats s{0, 0, 0};
+    s.slots_uploaded = (uint32_t)dirty.size();
+    if (!dirty.empty()) {
+      const float limit = 0.6f * 2.0f;
+      auto it = dirty.begin();
+      while (it != dirty.end()) {
+        const uint32_t start = *it;
+        uint32_t num_slots_r = 1, num_treelets = 1;
+        while (it != dirty.end()) {
+          const int dist = (int)*it - (int)start;
+          const float ratio = (float)(num_treelets + 1) / (float)(dist + 1);
+          if (ratio >= limit) {
+            num_slots_r = (uint32_t)dist + 1;
+            ++num_treelets;
+            ++it;
+          } else {
+            break;
+          }
+        }
+        ++s.copy_calls;
+        svo_status r = upload_range(start, num_slots_r, st, &s.bytes);
+        if (r != SVO_OK) return r;
+      }
+      svo_status r = upload_slot_table(st);
+      if (r != SVO_OK) return r;
+      dirty.clear();
+    }
+    if (stats) *stats = s;
+    return SVO_OK;
+  }
+};
+
+extern "C" svo_status svo_mm_create(const svo_scene* scene, uint64_t incore_bytes, int device, svo_mm** out) {
+  if (!scene || !out) { set_error("svo_mm_create: null argument"); return SVO_ERR_ARG; }
+  *out = nullptr;
+  const uint64_t tb = scene->treelet_bytes;
+  const uint64_t slots = incore_bytes / tb;
+  if (slots == 0) { set_error("svo_mm_create: incore buffer of %llu B holds no %llu B treelet", (unsigned long long)incore_bytes, (unsigned long long)tb); return SVO_ERR_ARG; }
+  const uint64_t pool_nodes = slots * (tb / 8);
+  if (pool_nodes >= 0xFFFFFFFFull) { set_error("svo_mm_create: pool of %llu nodes exceeds 32-bit node indices", (unsigned long long)pool_nodes); return SVO_ERR_ARG; }
+  if (device < -1) { set_error("svo_mm_create: device %d invalid", device); return SVO_ERR_ARG; }
+  if (device >= 0) {
+    int ndev = 0;
+    HIP_TRY(hipGetDeviceCount(&ndev));
+    if (device >= ndev) { set_error("svo_mm_create: device %d not present (%d devices)", device, ndev); return SVO_ERR_ARG; }
+  }
+  std::unique_ptr<svo_mm> mm(new svo_mm());
+  mm->scene = scene;
+  mm->N = (uint32_t)(tb / 8);
+  mm->num_slots = (uint32_t)slots;
+  mm->device = device;
+  try {
+    mm->incore.resize(pool_nodes);
+    mm->normals.resize(pool_nodes);
+    mm->colors.resize(pool_nodes);
+  } catch (const std::bad_alloc&) {
+    set_error("svo_mm_create: host incore buffer of %llu nodes", (unsigned long long)pool_nodes);
+    return SVO_ERR_NOMEM;
+  }
+  mm->reset_host();
+  svo_status r = mm->insert(0, 999.0f);  // root, SlotInfo(0, MAX_VISIBILITY, 999)
+  if (r != SVO_OK) return r;
+  if (mm->host_only()) {
+    mm->dirty.clear();
+    *out = mm.release();
+    return SVO_OK;
+  }
+  HIP_TRY(hipSetDevice(device));
+  if (hipMalloc(&mm->d_nodes, pool_nodes * 8) != hipSuccess || hipMalloc(&mm->d_colors, pool_nodes * 4) != hipSuccess ||
+      hipMalloc(&mm->d_normals, pool_nodes * 4) != hipSuccess || hipMalloc(&mm->d_slot_gid, slots * 4) != hipSuccess) {
+    set_error("svo_mm_create: hipMalloc of %llu B of pools failed", (unsigned long long)(pool_nodes * 16 + slots * 4));
+    return SVO_ERR_NOMEM;
+  }
+  HIP_TRY(hipEventCreateWithFlags(&mm->frames_ready, hipEventDisableTiming));
+  // initClBuffer: the whole incore buffer goes up once
+  r = mm->upload_range(0, mm->num_slots, nullptr, nullptr);
+  if (r != SVO_OK) return r;
+  r = mm->upload_slot_table(nullptr);
+  if (r != SVO_OK) return r;
+  mm->dirty.clear();
+  *out = mm.release();
+  return SVO_OK;
+}
+
+extern "C" void svo_mm_destroy(svo_mm* mm) { delete mm; }
+
+extern "C" svo_status svo_mm_reset(svo_mm* mm) {
+  if (!mm) { set_error("svo_mm_reset: null handle"); return SVO_ERR_ARG; }
+  mm->reset_host();
+  for (uint32_t s = 0; s < mm->num_slots; ++s) mm->mark(s);
+  return mm->insert(0, 999.0f);
+}
+
+extern "C" svo_status svo_mm_insert(svo_mm* mm, uint32_t gid, float error) {
+  if (!mm) { set_error("svo_mm_insert: null handle"); return SVO_ERR_ARG; }
+  return mm->insert(gid, error);
+}
+
+extern "C" svo_status svo_mm_remove(svo_mm* mm, uint32_t gid) {
+  if (!mm) { set_error("svo_mm_remove: null handle"); return SVO_ERR_ARG; }
+  if (gid == 0 || gid >= mm->scene->treelets.size()) { set_error("svo_mm_remove: gid %u cannot be removed", gid); return SVO_ERR_ARG; }
+  if (mm->treelet_slot[gid] < 0) { set_error("svo_mm_remove: treelet %u is not resident", gid); return SVO_ERR_STATE; }
+  mm->free_descendants(gid);
+  mm->remove_one(gid);
+  return SVO_OK;
+}
+
+extern "C" svo_status svo_mm_make_resident_all(svo_mm* mm) {
+  if (!mm) { set_error("svo_mm_make_resident_all: null handle"); return SVO_ERR_ARG; }
+  for (uint32_t g = 1; g < mm->scene->treelets.size(); ++g) {
+    if (mm->treelet_slot[g] >= 0) continue;
+    svo_status r = mm->insert(g, 0.0f);
+    if (r != SVO_OK) return r;
+  }
+  return SVO_OK;
+}
+
+extern "C" svo_status svo_mm_update_device(svo_mm* mm, void* stream, svo_upload_stats* stats) {
+  if (!mm) { set_error("svo_mm_update_device: null handle"); return SVO_ERR_ARG; }
+  if (!mm->host_only()) HIP_TRY(hipSetDevice(mm->device));
+  return mm->update_device((hipStream_t)stream, stats);
+}
+
+extern "C" svo_status svo_mm_get_info(const svo_mm* mm, svo_mm_info* out) {
+  if (!mm || !out) { set_error("svo_mm_get_info: null argument"); return SVO_ERR_ARG; }
+  out->num_slots = mm->num_slots;
+  out->nodes_per_treelet = mm->N;
+  out->free_slots = (uint32_t)mm->free_slots.size();
+  out->dirty_slots = (uint32_t)mm->dirty.size();
+  out->resident_treelets = mm->resident;
+  out->device = (uint32_t)mm->device;
+  out->pool_nodes = (uint64_t)mm->num_slots * mm->N;
+  out->device_bytes = mm->host_only() ? 0 : out->pool_nodes * 16 + (uint64_t)mm->num_slots * 4;
+  return SVO_OK;
+}
+
+extern "C" svo_status svo_mm_slot_treelet(const svo_mm* mm, uint32_t slot, int32_t* gid) {
+  if (!mm || !gid) { set_error("svo_mm_slot_treelet: null argument"); return SVO_ERR_ARG; }
+  if (slot >= mm->num_slots) { set_error("slot %u out of range (%u slots)", slot, mm->num_slots); return SVO_ERR_ARG; }
+  *gid = mm->slots[slot].gid;
+  return SVO_OK;
+}
+
+extern "C" svo_status svo_mm_treelet_slot(const svo_mm* mm, uint32_t gid, int32_t* slot) {
+  if (!mm || !slot) { set_error("svo_mm_treelet_slot: null argument"); return SVO_ERR_ARG; }
+  if (gid >= mm->treelet_slot.size()) { set_error("treelet gid %u out of range", gid); return SVO_ERR_ARG; }
+  *slot = mm->treelet_slot[gid];
+  return SVO_OK;
+}
+
+extern "C" svo_status svo_mm_read_host_incore(const svo_mm* mm, uint64_t first, uint64_t count, svo_node* nodes,
+                                              uint32_t* attribs) {
+  if (!mm) { set_error("svo_mm_read_host_incore: null handle"); return SVO_ERR_ARG; }
+  if (first > mm->incore.size() || count > mm->incore.size() - first) { set_error("incore range out of bounds"); return SVO_ERR_ARG; }
+  if (nodes) std::copy(mm->incore.begin() + first, mm->incore.begin() + first + count, nodes);
+  if (attribs)
+    for (uint64_t i = 0; i < count; ++i) {
+      attribs[2 * i] = mm->normals[first + i];
+      attribs[2 * i + 1] = mm->colors[first + i];
+    }
+  return SVO_OK;
+}
+
+extern "C" svo_status svo_mm_read_device_incore(const svo_mm* mm, uint64_t first, uint64_t count, svo_node* nodes,
+                                                uint32_t* attribs) {
+  if (!mm) { set_error("svo_mm_read_device_incore: null handle"); return SVO_ERR_ARG; }
+  if (first > mm->incore.size() || count > mm->incore.size() - first) { set_error("incore range out of bounds"); return SVO_ERR_ARG; }
+  if (mm->host_only()) { set_error("svo_mm_read_device_incore: handle has no device pools"); return SVO_ERR_STATE; }
+  HIP_TRY(hipSetDevice(mm->device));
+  if (nodes) HIP_TRY(hipMemcpy(nodes, mm->d_nodes + first, count * 8, hipMemcpyDeviceToHost));
+  if (attribs) {
+    std::vector<uint32_t> n(count), c(count);
+    HIP_TRY(hipMemcpy(n.data(), mm->d_normals + first, count * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(c.data(), mm->d_colors + first, count * 4, hipMemcpyDeviceToHost));
+    for (uint64_t i = 0; i < count; ++i) { attribs[2 * i] = n[i]; attribs[2 * i + 1] = c[i]; }
+  }
+  return SVO_OK;
+}
+
+// ------------------------------------------------------------ rendering
+
+namespace {
+
+svo_status make_pools(const svo_mm* mm, int policy, TracePools* P) {
+  const uint32_t D = mm->scene->max_depth;
+  P->nodes = mm->d_nodes;
+  P->colors = mm->d_colors;
+  P->slot_gid = mm->d_slot_gid;
+  P->nodes_per_treelet = mm->N;
+  P->slot_shift = ((mm->N & (mm->N - 1)) == 0) ? (uint32_t)__builtin_ctz(mm->N) : 32u;
+  P->num_slots = mm->num_slots;
+  P->pool_nodes = mm->num_slots * mm->N;
+  if (policy == SVO_POLICY_TREELET) {
+    // MAX_STACK_SIZE 13 / MAX_SVO_RAYCAST_DEPTH 12 (treeletRenderer_rgba.cl:7-8),
+    // widened to max(13, D+1) so that leaves deeper than 12 are reachable.
+    P->scale_max = std::max(13u, D + 1);
+    P->lod_cap = P->scale_max - 1;
+    P->max_loops = (P->scale_max + 1) * (P->scale_max + 1) * 5;
+  } else if (policy == SVO_POLICY_CPU3) {
+    // scaleMax = SVO depth, maxLoops = (D+1)^2 * 5
+    // (src/incore/CpuRaycasterSingleRay3.cpp:167,231)
+    P->scale_max = std::max(1u, D);
+    P->lod_cap = 0;
+    P->max_loops = (D + 1) * (D + 1) * 5;
+  } else {
+    set_error("unknown traversal policy %d", policy);
+    return SVO_ERR_ARG;
+  }
+  if (P->scale_max > 16) { set_error("scene depth %u exceeds the 16-level traversal stack", D); return SVO_ERR_ARG; }
+  return SVO_OK;
+}
+
+svo_status stage_frames(svo_mm* mm, const svo_camera* cams, uint32_t n, const svo_frame_out* outs, hipStream_t st,
+                        uint32_t* num_tiles) {
+  if (mm->frames_pending) {
+    HIP_TRY(hipEventSynchronize(mm->frames_ready));
+    mm->frames_pending = false;
+  }
+  if (n > mm->frames_cap) {
+    if (mm->d_frames) HIP_TRY(hipFree(mm->d_frames));
+    if (mm->h_frames) HIP_TRY(hipHostFree(mm->h_frames));
+    mm->d_frames = nullptr;
+    mm->h_frames = nullptr;
+    uint32_t cap = std::max(n, 64u);
+    HIP_TRY(hipMalloc(&mm->d_frames, cap * sizeof(TraceFrame)));
+    HIP_TRY(hipHostMalloc(&mm->h_frames, cap * sizeof(TraceFrame), hipHostMallocDefault));
+    mm->frames_cap = cap;
+  }
+  uint64_t tiles = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    const svo_camera& c = cams[i];
+    if (c.width == 0 || c.height == 0 || c.width > 16384 || c.height > 16384) {
+      set_error("camera %u: frame %ux%u out of range", i, c.width, c.height);
+      return SVO_ERR_ARG;
+    }
+    TraceFrame& F = mm->h_frames[i];
+    F.cam = c;
+    F.out = outs[i];
+    F.tile_begin = (uint32_t)tiles;
+    F.tiles_x = (c.width + 7) / 8;
+    F.pad[0] = F.pad[1] = 0;
+    tiles += (uint64_t)F.tiles_x * ((c.height + 7) / 8);
+  }
+  if (tiles > 0x7FFFFFFFull) { set_error("launch of %llu tiles too large", (unsigned long long)tiles); return SVO_ERR_ARG; }
+  HIP_TRY(hipMemcpyAsync(mm->d_frames, mm->h_frames, n * sizeof(TraceFrame), hipMemcpyHostToDevice, st));
+  HIP_TRY(hipEventRecord(mm->frames_ready, st));
+  mm->frames_pending = true;
+  *num_tiles = (uint32_t)tiles;
+  return SVO_OK;
+}
+
+}  // namespace
+
+extern "C" svo_status svo_render(svo_mm* mm, const svo_camera* cams, uint32_t n_cams, const svo_frame_out* outs,
+                                 int policy, void* stream) {
+  if (!mm || !cams || !outs || n_cams == 0) { set_error("svo_render: null argument or no camera"); return SVO_ERR_ARG; }
+  if (mm->host_only()) { set_error("svo_render: handle has no device pools"); return SVO_ERR_STATE; }
+  if (!mm->dirty.empty()) { set_error("svo_render: %zu slots not uploaded; call svo_mm_update_device first", mm->dirty.size()); return SVO_ERR_STATE; }
+  TracePools P;
+  svo_status r = make_pools(mm, policy, &P);
+  if (r != SVO_OK) return r;
+  HIP_TRY(hipSetDevice(mm->device));
+  hipStream_t st = (hipStream_t)stream;
+  uint32_t tiles = 0;
+  r = stage_frames(mm, cams, n_cams, outs, st, &tiles);
+  if (r != SVO_OK) return r;
+  int e = launch_trace(P, mm->d_frames, n_cams, tiles, policy, stream);
+  if (e != 0) { set_error("trace launch failed: %s", hipGetErrorString((hipError_t)e)); return SVO_ERR_HIP; }
+  return SVO_OK;
+}
+
+extern "C" svo_status svo_render_timed(svo_mm* mm, const svo_camera* cams, uint32_t n_cams, const svo_frame_out* outs,
+                                       int policy, void* stream, uint32_t iters, float* mean_ms) {
+  if (!mean_ms || iters == 0) { set_error("svo_render_timed: bad arguments"); return SVO_ERR_ARG; }
+  if (!mm || !cams || !outs || n_cams == 0) { set_error("svo_render_timed: null argument or no camera"); return SVO_ERR_ARG; }
+  if (mm->host_only()) { set_error("svo_render_timed: handle has no device pools"); return SVO_ERR_STATE; }
+  if (!mm->dirty.empty()) { set_error("svo_render_timed: slots not uploaded"); return SVO_ERR_STATE; }
+  TracePools P;
+  svo_status r = make_pools(mm, policy, &P);
+  if (r != SVO_OK) return r;
+  HIP_TRY(hipSetDevice(mm->device));
+  hipStream_t st = (hipStream_t)stream;
+  uint32_t tiles = 0;
+  r = stage_frames(mm, cams, n_cams, outs, st, &tiles);
+  if (r != SVO_OK) return r;
+  hipEvent_t a, b;
+  HIP_TRY(hipEventCreate(&a));
+  HIP_TRY(hipEventCreate(&b));
+  HIP_TRY(hipEventRecord(a, st));
+  for (uint32_t i = 0; i < iters; ++i) {
+    int e = launch_trace(P, mm->d_frames, n_cams, tiles, policy, stream);
+    if (e != 0) { set_error("trace launch failed: %s", hipGetErrorString((hipError_t)e)); return SVO_ERR_HIP; }
+  }
+  HIP_TRY(hipEventRecord(b, st));
+  HIP_TRY(hipEventSynchronize(b));
+  float ms = 0;
+  HIP_TRY(hipEventElapsedTime(&ms, a, b));
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  *mean_ms = ms / iters;
+  return SVO_OK;
+}
+
+// ------------------------------------------------------------ device utilities
+
+extern "C" svo_status svo_device_count(int* count) {
+  if (!count) { set_error("svo_device_count: null"); return SVO_ERR_ARG; }
+  hipError_t e = hipGetDeviceCount(count);
+  if (e != hipSuccess) { *count = 0; set_error("hipGetDeviceCount: %s", hipGetErrorString(e)); return SVO_ERR_HIP; }
+  return SVO_OK;
+}
+extern "C" svo_status svo_set_device(int device) { HIP_TRY(hipSetDevice(device)); return SVO_OK; }
+extern "C" svo_status svo_device_malloc(void** p, size_t bytes) {
+  if (!p) { set_error("svo_device_malloc: null"); return SVO_ERR_ARG; }
+  if (hipMalloc(p, bytes) != hipSuccess) { set_error("hipMalloc(%zu) failed", bytes); return SVO_ERR_NOMEM; }
+  return SVO_OK;
+}
+extern "C" svo_status svo_device_free(void* p) { HIP_TRY(hipFree(p)); return SVO_OK; }
+extern "C" svo_status svo_memcpy_h2d(void* d, const void* s, size_t n) { HIP_TRY(hipMemcpy(d, s, n, hipMemcpyHostToDevice)); return SVO_OK; }
+extern "C" svo_status svo_memcpy_d2h(void* d, const void* s, size_t n) { HIP_TRY(hipMemcpy(d, s, n, hipMemcpyDeviceToHost)); return SVO_OK; }
+extern "C" svo_status svo_memset_device(void* d, int v, size_t n) { HIP_TRY(hipMemset(d, v, n)); return SVO_OK; }
+extern "C" svo_status svo_stream_create(void** s) {
+  if (!s) { set_error("svo_stream_create: null"); return SVO_ERR_ARG; }
+  HIP_TRY(hipStreamCreateWithFlags((hipStream_t*)s, hipStreamNonBlocking));
+  return SVO_OK;
+}
+extern "C" svo_status svo_stream_destroy(void* s) { HIP_TRY(hipStreamDestroy((hipStream_t)s)); return SVO_OK; }
+extern "C" svo_status svo_stream_synchronize(void* s) { HIP_TRY(hipStreamSynchronize((hipStream_t)s)); return SVO_OK; }
+extern "C" svo_status svo_device_synchronize(void) { HIP_TRY(hipDeviceSynchronize()); return SVO_OK; }
+
+extern "C" svo_status svo_debug_ray_dirs(const svo_camera* cam, float* dirs_dev, void* stream) {
+  if (!cam || !dirs_dev) { set_error("svo_debug_ray_dirs: null argument"); return SVO_ERR_ARG; }
+  int e = launch_ray_dirs(*cam, dirs_dev, stream);
+  if (e != 0) { set_error("ray-direction launch failed: %s", hipGetErrorString((hipError_t)e)); return SVO_ERR_HIP; }
+  return SVO_OK;
+}

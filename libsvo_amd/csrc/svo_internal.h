@@ -1,0 +1,88 @@
+// Internal C++ declarations shared by the host translation units of
+// libsvo_hip.so.  Not installed; the public surface is include/svo_hip.h and
+// the C++ wrappers in libsvo_amd/csrc/svo/.
+#pragma once
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/svo_hip.h"
+
+namespace svo_impl {
+
+// ------------------------------------------------------------ errors
+
+void set_error(const char* fmt, ...) __attribute__((format(printf, 1, 2)));
+void clear_error();
+
+// ------------------------------------------------------------ node masks
+// Bit layout of CpuSvoNode::_masks (src/CpuSvoNode.h:46-48): leaf mask in
+// bits 0-7, valid mask in bits 8-15.
+constexpr uint32_t kLeafShift = 0;
+constexpr uint32_t kValidShift = 8;
+
+inline bool valid_flag(uint32_t masks, uint32_t c) { return (masks >> (kValidShift + c)) & 1u; }
+inline bool leaf_flag(uint32_t masks, uint32_t c) { return (masks >> (kLeafShift + c)) & 1u; }
+// getNthChildIndex (src/CpuSvoNode.cpp:145-152): first + number of valid
+// children with a smaller child index, modulo 2^32.
+inline uint32_t nth_child(uint32_t masks, uint32_t first, uint32_t c) {
+  return first + (uint32_t)__builtin_popcount((masks >> kValidShift) & ((1u << c) - 1u));
+}
+
+// ------------------------------------------------------------ attributes
+// Packed colour r<<24|g<<16|b<<8|a, unpacked by unpackRgbaFromUnsigned
+// (opencl/treeletRenderer_rgba.cl:183-195).
+inline uint32_t pack_rgba(uint32_t r, uint32_t g, uint32_t b, uint32_t a) {
+  return (r << 24) | (g << 16) | (b << 8) | a;
+}
+
+// ------------------------------------------------------------ hashing
+inline uint64_t mix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+inline uint64_t hash_xyz(uint64_t seed, uint32_t i, uint32_t j, uint32_t k) {
+  return mix64(mix64(seed) ^ (((uint64_t)i << 42) | ((uint64_t)j << 21) | (uint64_t)k));
+}
+
+// ------------------------------------------------------------ morton
+// Child index of a node inside its parent is 4x+2y+z (x = 1 for the
+// positive half), opencl/treeletRenderer_rgba.cl:289-291.  A Morton code of
+// depth D holds D such triplets, the root's child first.
+uint64_t morton_encode(uint32_t i, uint32_t j, uint32_t k, uint32_t depth);
+void morton_decode(uint64_t code, uint32_t depth, uint32_t* i, uint32_t* j, uint32_t* k);
+
+// In-place LSD radix sort of 64-bit keys whose set bits lie below `bits`.
+void radix_sort_u64(std::vector<uint64_t>& keys, uint32_t bits);
+
+// ------------------------------------------------------------ scene
+struct TreeletData {
+  svo_treelet_header h{};
+  std::vector<svo_node> nodes;     // h.num_nodes
+  std::vector<uint32_t> attribs;   // 2 * h.num_nodes: {normal, color}
+};
+
+}  // namespace svo_impl
+
+struct svo_scene {
+  uint32_t treelet_bytes = 0;
+  uint32_t nodes_per_treelet = 0;
+  uint32_t max_depth = 0;
+  uint64_t tree_nodes = 0;
+  uint64_t tree_leaves = 0;
+  std::vector<svo_impl::TreeletData> treelets;
+};
+
+namespace svo_impl {
+
+// Builds the scene from sorted, unique leaf codes at `depth` and two
+// attribute words per leaf.  Consumes `codes` (cleared on return).
+svo_status build_scene(uint32_t depth, std::vector<uint64_t>& codes,
+                       std::vector<uint32_t>& leaf_attribs,
+                       uint32_t treelet_bytes, uint32_t flags, svo_scene** out);
+
+}  // namespace svo_impl

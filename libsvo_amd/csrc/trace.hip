@@ -1,0 +1,303 @@
+// Serialized-SVO treelet traversal for gfx950.
+//
+// One ray per lane, one 8x8 pixel tile per wavefront, four wavefronts per
+// workgroup.  The per-ray stack lives in LDS ([level][lane], conflict-free),
+// holding only what cannot be recomputed:
+//   * the parent's child base (node index + first-child offset) and masks;
+//   * the child index taken from that parent (3 bits, packed above the masks).
+// A parent's centre, its tmax and its tmin after the pop are recomputed
+// bit-exactly instead of stored:
+//   * centres are dyadic, so parent = child - (-0.5 + bit) * 2^(s - scaleMax)
+//     is exact;
+//   * a node's tmax is the tcMax computed when it was pushed, a pure function
+//     of (ray, centre, size) evaluated with the same FP32 operations;
+//   * after a pop the parent's tmin equals the popped child's tmax, because
+//     the reference sets parentTMin = tcMax right before the push
+//     (opencl/treeletRenderer_rgba.cl:347-357,
+//     src/incore/CpuRaycasterSingleRay3.cpp:355-363) and nothing changes it
+//     until the pop.
+// Every floating-point expression follows the reference's operation order
+// and is compiled with -ffp-contract=off, so the kernel reproduces the CPU
+// restatement bit for bit.
+#include <hip/hip_runtime.h>
+
+#include "trace.h"
+
+namespace svo_impl {
+namespace {
+
+constexpr int kWaves = 4;
+constexpr int kBlock = 64 * kWaves;
+constexpr int kMaxScale = 16;  // stack levels; scaleMax <= 16
+
+// OpenCL max/min as specified for floats: max -> y if x < y else x,
+// min -> y if y < x else x.
+__device__ __forceinline__ float cl_max(float x, float y) { return (x < y) ? y : x; }
+__device__ __forceinline__ float cl_min(float x, float y) { return (y < x) ? y : x; }
+// OpenCL sign(): 1, -1, signed zero preserved, 0 for NaN.
+__device__ __forceinline__ float cl_sign(float x) {
+  return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : (x == x ? x : 0.0f));
+}
+// 2^k for k in [-126, 127], exact.
+__device__ __forceinline__ float exp2i(int k) { return __int_as_float((127 + k) << 23); }
+
+// intersectAABB against [-0.5,0.5]^3 (opencl/treeletRenderer_rgba.cl:100-178).
+__device__ __forceinline__ bool root_box(const float o[3], const float d[3], float& t_min, float& t_max) {
+  float t_near = -10000000.0f, t_far = 10000000.0f;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    if (d[i] == 0.0f && (-0.5f > o[i] || 0.5f < o[i])) return false;
+    float t1 = (-0.5f - o[i]) / d[i];
+    float t2 = (0.5f - o[i]) / d[i];
+    if (t1 > t2) { float tmp = t1; t1 = t2; t2 = tmp; }
+    if (t1 > t_near) t_near = t1;
+    if (t2 < t_far) t_far = t2;
+    if (t_near > t_far) return false;
+    if (t_far < 0.0f) return false;
+  }
+  t_min = t_near;
+  t_max = t_far;
+  return true;
+}
+
+// Ray of pixel (x,y) as in renderToBuffer (treeletRenderer_rgba.cl:627-633):
+// no half-pixel offset, y flipped; normalize(v) defined as v / sqrt(v.v).
+__device__ __forceinline__ void pixel_ray(const svo_camera& c, uint32_t x, uint32_t y, float d[3]) {
+  const float fx = (float)x;
+  const float fy = (float)(c.height - y);
+  float v[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float p = (c.far_lower_left[a] + c.pixel_h[a] * fx) + c.pixel_v[a] * fy;
+    v[a] = p - c.position[a];
+  }
+  const float len = sqrtf((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) d[a] = v[a] / len;
+}
+
+struct Hit {
+  uint32_t node;  // 0xFFFFFFFF = miss
+  float t;
+  uint32_t pushes;
+};
+
+// tcMax of the node centred at c with half-edge h (the value computed when
+// the node was pushed).
+__device__ __forceinline__ float node_tmax(const float c[3], float h, const float rc[3], const float mo[3]) {
+  float t1[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float lo = rc[a] * (c[a] - h) + mo[a];
+    const float hi = rc[a] * (c[a] + h) + mo[a];
+    t1[a] = (lo > hi) ? lo : hi;  // swapToIncreasing keeps the larger in t1
+  }
+  return cl_min(t1[0], cl_min(t1[1], t1[2]));
+}
+
+// POLICY 0: treeletRenderer_rgba.cl::sample (:201-382).
+// POLICY 2: CpuRaycasterSingleRay3::traversSvo
+//           (src/incore/CpuRaycasterSingleRay3.cpp:158-387).
+template <int POLICY>
+__device__ Hit traverse(const TracePools& S, const float o[3], float d[3], float ratio, uint32_t* s_base,
+                        uint32_t* s_mask, uint32_t tid) {
+  Hit r{0xFFFFFFFFu, __int_as_float(0x7F800000), 0u};
+  float t_min, t_max;
+  if (!root_box(o, d, t_min, t_max)) return r;
+  r.t = t_max;
+
+  const float eps = 0.00001f;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    if (POLICY == 0) {
+      if (fabsf(d[a]) < 0.0001f) d[a] = 0.0001f * cl_sign(d[a]);
+    } else {
+      if (fabsf(d[a]) < eps) d[a] = eps * cl_sign(d[a]) * 10.0f;
+    }
+  }
+  float rc[3], mo[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    rc[a] = 1.0f / d[a];
+    mo[a] = -o[a] * rc[a];
+  }
+
+  const int smax = (int)S.scale_max;
+  int scale = smax - 1;
+  float c[3] = {0.0f, 0.0f, 0.0f};
+  float tmin = t_min, tmax = t_max;
+  const uint64_t root = S.nodes[0];
+  uint32_t base = (uint32_t)root;  // node index 0 + first child
+  uint32_t masks = (uint32_t)(root >> 32);
+  uint32_t iter = 0;
+
+  for (;;) {
+    const float se = exp2i(scale - smax);  // scale_exp2
+    const float h = se * 0.5f;             // childSizeHalf
+    ++iter;
+    if (iter == S.max_loops) {
+      if (POLICY == 0) {  // maxLoops returns the current parent as a hit
+        uint32_t own = 0;
+        if (scale + 1 < smax) {
+          const uint32_t b = s_base[(scale + 1) * kBlock + tid];
+          const uint32_t m = s_mask[(scale + 1) * kBlock + tid];
+          const uint32_t cc = m >> 16;
+          own = b + (uint32_t)__popc((m >> 8) & ((1u << cc) - 1u));
+        }
+        r.node = own;
+        r.t = tmin;
+      }
+      return r;
+    }
+    bool pop;
+    if (POLICY == 0) pop = (tmax < 0.0f) || !(fabsf(tmin - tmax) > eps);
+    else pop = (tmax < 0.0f) || (fabsf(tmin - tmax) < eps) || !(tmin < tmax);
+
+    if (!pop) {
+      const float te = tmin + eps;
+      const float ex = (o[0] + te * d[0]) - c[0];
+      const float ey = (o[1] + te * d[1]) - c[1];
+      const float ez = (o[2] + te * d[2]) - c[2];
+      const uint32_t ci = (ex > 0.0f ? 4u : 0u) | (ey > 0.0f ? 2u : 0u) | (ez > 0.0f ? 1u : 0u);
+      float cc[3];
+      cc[0] = (-0.5f + (float)((ci >> 2) & 1u)) * se + c[0];
+      cc[1] = (-0.5f + (float)((ci >> 1) & 1u)) * se + c[1];
+      cc[2] = (-0.5f + (float)(ci & 1u)) * se + c[2];
+      float t0[3], t1[3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const float lo = rc[a] * (cc[a] - h) + mo[a];
+        const float hi = rc[a] * (cc[a] + h) + mo[a];
+        const bool sw = lo > hi;
+        t0[a] = sw ? hi : lo;
+        t1[a] = sw ? lo : hi;
+      }
+      const float tc_min = cl_max(t0[0], cl_max(t0[1], t0[2]));
+      const float tc_max = cl_min(t1[0], cl_min(t1[1], t1[2]));
+
+      if ((masks >> (8u + ci)) & 1u) {
+        const uint32_t child = base + (uint32_t)__popc((masks >> 8) & ((1u << ci) - 1u));
+        if ((masks >> ci) & 1u) {  // leaf
+          r.node = child;
+          r.t = tmin;
+          return r;
+        }
+        bool stop;
+        if (POLICY == 0) stop = (ratio * tc_min > se) || (S.lod_cap != 0 && (uint32_t)(smax - scale) == S.lod_cap);
+        else stop = (ratio * tc_max > se) || scale == 0;
+        if (stop) {
+          r.node = child;
+          r.t = tmin;
+          return r;
+        }
+        if (child >= S.pool_nodes) {  // corrupt offset: never read outside the pool
+          r.node = 0xFFFFFFFFu;
+          return r;
+        }
+        // push
+        ++r.pushes;
+        s_base[scale * kBlock + tid] = base;
+        s_mask[scale * kBlock + tid] = (masks & 0xFFFFu) | (ci << 16);
+        --scale;
+        c[0] = cc[0]; c[1] = cc[1]; c[2] = cc[2];
+        tmin = tc_min;
+        tmax = tc_max;
+        const uint64_t nd = S.nodes[child];
+        base = child + (uint32_t)nd;
+        masks = (uint32_t)(nd >> 32);
+      } else {
+        tmin = tc_max;  // advance
+      }
+    } else {
+      ++scale;
+      if (scale >= smax) return r;  // left the root: miss, t = root tMax
+      const float t_child = tmax;
+      base = s_base[scale * kBlock + tid];
+      const uint32_t m = s_mask[scale * kBlock + tid];
+      masks = m & 0xFFFFu;
+      const uint32_t ci = m >> 16;
+      const float sp = exp2i(scale - smax);
+      c[0] = c[0] - (-0.5f + (float)((ci >> 2) & 1u)) * sp;
+      c[1] = c[1] - (-0.5f + (float)((ci >> 1) & 1u)) * sp;
+      c[2] = c[2] - (-0.5f + (float)(ci & 1u)) * sp;
+      tmin = t_child;
+      tmax = (scale == smax - 1) ? t_max : node_tmax(c, sp, rc, mo);
+    }
+  }
+}
+
+template <int POLICY>
+__global__ __launch_bounds__(kBlock) void trace_kernel(TracePools S, const TraceFrame* __restrict__ frames,
+                                                       uint32_t num_frames, uint32_t num_tiles) {
+  __shared__ uint32_t s_base[kMaxScale * kBlock];
+  __shared__ uint32_t s_mask[kMaxScale * kBlock];
+  const uint32_t tid = threadIdx.x;
+  const uint32_t lane = tid & 63u;
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const uint32_t tile = blockIdx.x * kWaves + wave;
+  if (tile >= num_tiles) return;
+
+  uint32_t lo = 0, hi = num_frames;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (frames[mid].tile_begin <= tile) lo = mid;
+    else hi = mid;
+  }
+  const TraceFrame& F = frames[lo];
+  const uint32_t lt = tile - F.tile_begin;
+  const uint32_t x = (lt % F.tiles_x) * 8u + (lane & 7u);
+  const uint32_t y = (lt / F.tiles_x) * 8u + (lane >> 3);
+  if (x >= F.cam.width || y >= F.cam.height) return;
+
+  const float o[3] = {F.cam.position[0], F.cam.position[1], F.cam.position[2]};
+  float d[3];
+  pixel_ray(F.cam, x, y, d);
+  const Hit hit = traverse<POLICY>(S, o, d, F.cam.t_scale_ratio, s_base, s_mask, tid);
+
+  const size_t pix = (size_t)y * F.cam.width + x;
+  uint32_t tree = 0xFFFFFFFFu, rgba = 0x334D33FFu;
+  if (hit.node != 0xFFFFFFFFu && hit.node < S.pool_nodes) {
+    const uint32_t slot = (S.slot_shift < 32) ? (hit.node >> S.slot_shift) : (hit.node / S.nodes_per_treelet);
+    tree = (uint32_t)S.slot_gid[slot];
+    rgba = S.colors[hit.node] | 0xFFu;
+  }
+  if (F.out.t) F.out.t[pix] = hit.t;
+  if (F.out.node) F.out.node[pix] = hit.node;
+  if (F.out.treelet) F.out.treelet[pix] = tree;
+  if (F.out.rgba) F.out.rgba[pix] = rgba;
+  if (F.out.pushes) F.out.pushes[pix] = hit.pushes;
+}
+
+__global__ void ray_dir_kernel(svo_camera cam, float* dirs) {
+  const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t y = blockIdx.y;
+  if (x >= cam.width || y >= cam.height) return;
+  float d[3];
+  pixel_ray(cam, x, y, d);
+  const size_t p = (size_t)y * cam.width + x;
+  dirs[3 * p + 0] = d[0];
+  dirs[3 * p + 1] = d[1];
+  dirs[3 * p + 2] = d[2];
+}
+
+}  // namespace
+
+int launch_trace(const TracePools& pools, const TraceFrame* frames_dev, uint32_t num_frames, uint32_t num_tiles,
+                 int policy, void* stream) {
+  if (num_tiles == 0) return 0;
+  const dim3 grid((num_tiles + kWaves - 1) / kWaves), block(kBlock);
+  hipStream_t st = (hipStream_t)stream;
+  if (policy == SVO_POLICY_CPU3)
+    hipLaunchKernelGGL(trace_kernel<2>, grid, block, 0, st, pools, frames_dev, num_frames, num_tiles);
+  else
+    hipLaunchKernelGGL(trace_kernel<0>, grid, block, 0, st, pools, frames_dev, num_frames, num_tiles);
+  return (int)hipGetLastError();
+}
+
+int launch_ray_dirs(const svo_camera& cam, float* dirs_dev, void* stream) {
+  const dim3 grid((cam.width + 63) / 64, cam.height), block(64);
+  hipLaunchKernelGGL(ray_dir_kernel, grid, block, 0, (hipStream_t)stream, cam, dirs_dev);
+  return (int)hipGetLastError();
+}
+
+}  // namespace svo_impl

@@ -1,0 +1,605 @@
+// CPU oracle for the serialized-SVO treelet ray caster.
+//
+// TEST INFRASTRUCTURE ONLY.  This file restates the reference's algorithms on
+// the CPU so that the HIP path can be checked against them; it is loaded by
+// the tests, by __graft_entry__.smoke() as the checker and by bench.py's
+// CPU-baseline leg, and never by the product library.
+//
+// It is written independently of libsvo_amd/csrc: its own .svo/.ia reader,
+// its own incore-buffer stitching and a traversal that keeps the reference's
+// full explicit stack (index, tmin, tmax, centre per level) instead of the
+// GPU kernel's recomputed stack.
+//
+// Restated reference functions (paths relative to the libsvo tree):
+//   read_bundle      TreeletMemoryManager::loadFromFile + Treelet::loadFromFile
+//                    (src/TreeletMemoryManager.cpp:229-291,
+//                    src/Treelet.cpp:459-478); .ia is this project's layout
+//   make_incore      resetIncoreBuffer + insertTreeletIntoIncoreBuffer for
+//                    every treelet in gid order
+//                    (src/TreeletMemoryManager.cpp:303-380,455-548)
+//   intersect_aabb   opencl/treeletRenderer_rgba.cl:100-178
+//   pixel_ray        renderToBuffer ray set-up, treeletRenderer_rgba.cl:627-633
+//                    (normalize(v) taken as v / sqrt(v.v))
+//   sample_treelet   treeletRenderer_rgba.cl:201-382 (policy A)
+//   travers_cpu3     src/incore/CpuRaycasterSingleRay3.cpp:158-387 (policy C),
+//                    FP32 arithmetic, relative child indexing on the incore
+//                    buffer, the same root-box routine as policy A
+//   get_color        treeletRenderer_rgba.cl:183-195,399-404,645-655
+//
+// Pinning status of each function is recorded in tests/test_oracle_cpu.py;
+// the boundary-ray flag is this project's definition (a ray is flagged when a
+// child-selection step comes within 1e-6 of a face or of the pop epsilon).
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include <omp.h>
+
+namespace {
+
+struct Node {
+  uint32_t first;
+  uint32_t masks;
+};
+
+struct Header {
+  uint32_t num_nodes, num_leaves, mem_size, gid, parent_gid, leaf_pos, leaf_idx, leaf_parent_pos;
+};
+
+struct Treelet {
+  Header h;
+  std::vector<Node> nodes;      // padded to mem_size / 8
+  std::vector<uint32_t> attr;   // normal, colour per node (padded)
+};
+
+// Same memory layout as the library's svo_camera (80 bytes).
+struct Camera {
+  float far_lower_left[4];
+  float pixel_h[4];
+  float pixel_v[4];
+  float position[4];
+  float t_scale_ratio;
+  uint32_t width, height, reserved;
+};
+static_assert(sizeof(Camera) == 80, "camera layout");
+
+}  // namespace
+
+struct oracle_scene {
+  uint32_t treelet_bytes = 0, N = 0, max_depth = 0;
+  std::vector<Treelet> treelets;
+  // incore buffer
+  uint32_t num_slots = 0;
+  std::vector<Node> incore;
+  std::vector<uint32_t> colors, normals;
+  std::vector<int32_t> slot_gid;
+};
+
+namespace {
+
+bool read_u32(FILE* f, uint32_t* v) { return fread(v, 4, 1, f) == 1; }
+
+int read_bundle(const std::string& base, oracle_scene* s) {
+  FILE* f = fopen((base + ".svo").c_str(), "rb");
+  if (!f) return 2;
+  uint32_t nt = 0;
+  if (!read_u32(f, &nt) || !read_u32(f, &s->treelet_bytes) || s->treelet_bytes < 128) { fclose(f); return 3; }
+  s->N = s->treelet_bytes / 8;
+  s->treelets.resize(nt);
+  for (uint32_t t = 0; t < nt; ++t) {
+    Treelet& T = s->treelets[t];
+    if (fread(&T.h, sizeof(Header), 1, f) != 1 || T.h.num_nodes > s->N) { fclose(f); return 3; }
+    T.nodes.assign(s->N, Node{0, 0});
+    if (fread(T.nodes.data(), 8, T.h.num_nodes, f) != T.h.num_nodes) { fclose(f); return 3; }
+  }
+  fclose(f);
+  f = fopen((base + ".ia").c_str(), "rb");
+  if (!f) return 2;
+  uint32_t na = 0;
+  if (!read_u32(f, &na) || na < nt) { fclose(f); return 3; }
+  for (uint32_t t = 0; t < nt; ++t) {
+    Treelet& T = s->treelets[t];
+    uint32_t np = 0, w = 0;
+    if (!read_u32(f, &np) || !read_u32(f, &w) || w != 2 || np != T.h.num_nodes) { fclose(f); return 3; }
+    T.attr.assign(2 * (size_t)s->N, 0u);
+    if (fread(T.attr.data(), 4, 2 * (size_t)np, f) != 2 * (size_t)np) { fclose(f); return 3; }
+  }
+  fclose(f);
+  return 0;
+}
+
+// Depth of the deepest leaf, walking each treelet from its root.
+uint32_t bundle_depth(const oracle_scene& s) {
+  std::vector<uint32_t> root_depth(s.treelets.size(), 0);
+  std::vector<std::vector<uint32_t>> kids(s.treelets.size());
+  for (size_t u = 1; u < s.treelets.size(); ++u)
+    if (s.treelets[u].h.parent_gid < u) kids[s.treelets[u].h.parent_gid].push_back((uint32_t)u);
+  uint32_t maxd = 0;
+  for (size_t t = 0; t < s.treelets.size(); ++t) {
+    const Treelet& T = s.treelets[t];
+    std::vector<uint32_t> d(T.h.num_nodes, 0);
+    d[0] = root_depth[t];
+    for (uint32_t i = 0; i < T.h.num_nodes; ++i) {
+      uint32_t idx = T.nodes[i].first;
+      for (uint32_t c = 0; c < 8; ++c) {
+        if (!(T.nodes[i].masks & (1u << (8 + c)))) continue;
+        const uint32_t ch = i + idx;
+        ++idx;
+        if (ch < T.h.num_nodes) {
+          d[ch] = d[i] + 1;
+          if ((T.nodes[i].masks & (1u << c)) && d[ch] > maxd) maxd = d[ch];
+        }
+      }
+    }
+    for (uint32_t u : kids[t])
+      if (s.treelets[u].h.leaf_pos < d.size()) root_depth[u] = d[s.treelets[u].h.leaf_pos];
+  }
+  return maxd;
+}
+
+int make_incore(oracle_scene* s, uint64_t incore_bytes) {
+  const uint64_t slots = incore_bytes / s->treelet_bytes;
+  if (slots < s->treelets.size()) return 7;
+  s->num_slots = (uint32_t)slots;
+  const uint64_t n = slots * s->N;
+  s->incore.assign(n, Node{0, 0});
+  s->colors.assign(n, 0u);
+  s->normals.assign(n, 0u);
+  s->slot_gid.assign(slots, -1);
+  std::vector<int32_t> slot_of(s->treelets.size(), -1);
+  // the free-slot stack yields 0, 1, 2, ... ; inserting in gid order puts
+  // treelet g into slot g
+  for (uint32_t g = 0; g < s->treelets.size(); ++g) {
+    const Treelet& T = s->treelets[g];
+    const uint64_t pos = (uint64_t)g * s->N;
+    slot_of[g] = (int32_t)g;
+    s->slot_gid[g] = (int32_t)g;
+    for (uint32_t i = 0; i < s->N; ++i) {
+      s->incore[pos + i] = T.nodes[i];
+      s->normals[pos + i] = T.attr[2 * (size_t)i];
+      s->colors[pos + i] = T.attr[2 * (size_t)i + 1];
+    }
+    if (g == 0) continue;
+    const uint64_t pslot = (uint64_t)slot_of[T.h.parent_gid];
+    const uint64_t leaf = pslot * s->N + T.h.leaf_pos;
+    const uint64_t leaf_parent = pslot * s->N + T.h.leaf_parent_pos;
+    s->incore[leaf] = s->incore[pos];
+    s->incore[leaf_parent].masks &= ~(1u << T.h.leaf_idx);
+    s->incore[leaf].first = (uint32_t)((int32_t)(pos + 1) - (int32_t)leaf);
+  }
+  return 0;
+}
+
+// ------------------------------------------------------------ traversal
+
+struct Buffers {
+  const Node* nodes;
+  const uint32_t* colors;
+  const int32_t* slot_gid;
+  uint64_t pool;
+  uint32_t N;
+  uint32_t num_slots;
+  uint32_t depth;
+};
+
+struct Result {
+  bool hit;
+  uint32_t node;
+  uint32_t depth;
+  float t;
+  uint32_t loops;
+  float center[3];
+  uint32_t pushes;
+  bool boundary;
+};
+
+inline float cl_max(float x, float y) { return (x < y) ? y : x; }
+inline float cl_min(float x, float y) { return (y < x) ? y : x; }
+inline float cl_sign(float x) { return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : (x == x ? x : 0.0f)); }
+inline void swap_to_increasing(float& a, float& b) {
+  if (a > b) { float t = a; a = b; b = t; }
+}
+inline bool valid_flag(uint32_t masks, uint32_t i) { return (masks & (1u << (i + 8))) != 0; }
+inline bool leaf_flag(uint32_t masks, uint32_t i) { return (masks & (1u << i)) != 0; }
+inline uint32_t nth_child_idx(uint32_t masks, uint32_t first, uint32_t child) {
+  uint32_t index = first;
+  for (uint32_t i = 0; i != child; ++i) index += valid_flag(masks, i);
+  return index;
+}
+
+bool intersect_aabb(const float o[3], const float d[3], float* t_min, float* t_max) {
+  float t_near = -10000000.0f, t_far = 10000000.0f;
+  const float r = 0.5f;
+  for (int i = 0; i != 3; ++i) {
+    if (d[i] == 0 && (-r > o[i] || r < o[i])) return false;
+    float t1 = (-r - o[i]) / d[i];
+    float t2 = (r - o[i]) / d[i];
+    if (t1 > t2) { float tmp = t1; t1 = t2; t2 = tmp; }
+    if (t1 > t_near) t_near = t1;
+    if (t2 < t_far) t_far = t2;
+    if (t_near > t_far) return false;
+    if (t_far < 0.0f) return false;
+  }
+  *t_min = t_near;
+  *t_max = t_far;
+  return true;
+}
+
+void pixel_ray(const Camera& c, uint32_t x, uint32_t y, float d[3]) {
+  float v[3];
+  for (int a = 0; a < 3; ++a) {
+    float p = c.far_lower_left[a] + c.pixel_h[a] * (float)x;
+    p = p + c.pixel_v[a] * (float)(c.height - y);
+    v[a] = p - c.position[a];
+  }
+  float len = std::sqrt((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]);
+  for (int a = 0; a < 3; ++a) d[a] = v[a] / len;
+}
+
+// Boundary test of one child-selection step (this project's definition).
+inline bool near_boundary(const float e[3], float tmin, float tmax, float eps, const float t0[3], const float t1[3]) {
+  const float b = 1e-6f;
+  if (std::fabs(e[0]) < b || std::fabs(e[1]) < b || std::fabs(e[2]) < b) return true;
+  if (std::fabs(std::fabs(tmin - tmax) - eps) < b) return true;
+  // two largest entry planes / two smallest exit planes
+  float a0 = t0[0], a1 = t0[1], a2 = t0[2];
+  if (a0 < a1) std::swap(a0, a1);
+  if (a1 < a2) std::swap(a1, a2);
+  if (a0 < a1) std::swap(a0, a1);
+  if (std::fabs(a0 - a1) < b) return true;
+  float c0 = t1[0], c1 = t1[1], c2 = t1[2];
+  if (c0 > c1) std::swap(c0, c1);
+  if (c1 > c2) std::swap(c1, c2);
+  if (c0 > c1) std::swap(c0, c1);
+  return std::fabs(c0 - c1) < b;
+}
+
+struct StackElement {
+  uint32_t parent_node_index;
+  float parent_tmin;
+  float parent_tmax;
+  float center[3];
+};
+
+constexpr int kMaxStack = 24;
+
+// Policy A: treeletRenderer_rgba.cl::sample, scaleMax = max(13, D+1).
+void sample_treelet(const Buffers& B, const float o[3], float d[3], float ratio, Result* r) {
+  r->hit = false;
+  r->loops = 0;
+  r->pushes = 0;
+  r->boundary = false;
+  r->node = 0xFFFFFFFFu;
+  r->depth = 0;
+  r->t = INFINITY;
+  r->center[0] = r->center[1] = r->center[2] = 0.0f;
+  float t_min, t_max;
+  if (!intersect_aabb(o, d, &t_min, &t_max)) return;
+
+  const int scale_max = (int)std::max(13u, B.depth + 1);
+  const int max_depth_cap = scale_max - 1;  // MAX_SVO_RAYCAST_DEPTH
+  int scale = scale_max - 1;
+  float scale_exp2 = 0.5f;
+  const float min_normal = 0.0001f;
+  const float epsilon = 0.00001f;
+  StackElement stack[kMaxStack];
+  stack[scale] = StackElement{0, t_min, t_max, {0.0f, 0.0f, 0.0f}};
+  for (int a = 0; a < 3; ++a)
+    if (std::fabs(d[a]) < min_normal) d[a] = min_normal * cl_sign(d[a]);
+  const float dx_r = 1.0f / d[0], mpx = -o[0] * dx_r;
+  const float dy_r = 1.0f / d[1], mpy = -o[1] * dy_r;
+  const float dz_r = 1.0f / d[2], mpz = -o[2] * dz_r;
+  unsigned while_counter = 0;
+  const unsigned max_loops = (scale_max + 1) * (scale_max + 1) * 5;
+
+  while (scale < scale_max) {
+    StackElement* parent = &stack[scale];
+    scale_exp2 = std::ldexp(1.0f, scale - scale_max);
+    const float child_size_half = scale_exp2 * 0.5f;
+    ++while_counter;
+    if (while_counter == max_loops) {
+      r->hit = true;
+      r->node = parent->parent_node_index;
+      r->depth = scale_max - scale;
+      r->t = parent->parent_tmin;
+      r->loops = while_counter;
+      std::memcpy(r->center, parent->center, sizeof r->center);
+      return;
+    }
+    if (parent->parent_tmax < 0.0f) { ++scale; continue; }
+    if (std::fabs(parent->parent_tmin - parent->parent_tmax) > epsilon) {
+      float e[3];
+      for (int a = 0; a < 3; ++a) e[a] = (o[a] + (parent->parent_tmin + epsilon) * d[a]) - parent->center[a];
+      const int child_idx = 4 * (e[0] > 0.0f) + 2 * (e[1] > 0.0f) + (e[2] > 0.0f);
+      float cc[3];
+      cc[0] = (-0.5f + (bool)(child_idx & 4)) * scale_exp2 + parent->center[0];
+      cc[1] = (-0.5f + (bool)(child_idx & 2)) * scale_exp2 + parent->center[1];
+      cc[2] = (-0.5f + (bool)(child_idx & 1)) * scale_exp2 + parent->center[2];
+      float tx0 = dx_r * (cc[0] - child_size_half) + mpx;
+      float tx1 = dx_r * (cc[0] + child_size_half) + mpx;
+      swap_to_increasing(tx0, tx1);
+      float ty0 = dy_r * (cc[1] - child_size_half) + mpy;
+      float ty1 = dy_r * (cc[1] + child_size_half) + mpy;
+      swap_to_increasing(ty0, ty1);
+      float tz0 = dz_r * (cc[2] - child_size_half) + mpz;
+      float tz1 = dz_r * (cc[2] + child_size_half) + mpz;
+      swap_to_increasing(tz0, tz1);
+      const float tc_min = cl_max(tx0, cl_max(ty0, tz0));
+      const float tc_max = cl_min(tx1, cl_min(ty1, tz1));
+      {
+        const float t0[3] = {tx0, ty0, tz0}, t1[3] = {tx1, ty1, tz1};
+        if (near_boundary(e, parent->parent_tmin, parent->parent_tmax, epsilon, t0, t1)) r->boundary = true;
+      }
+      const uint64_t pidx = parent->parent_node_index;
+      if (pidx >= B.pool) return;  // corrupt index: miss
+      const Node pn = B.nodes[pidx];
+      if (valid_flag(pn.masks, child_idx)) {
+        const uint32_t child = parent->parent_node_index + nth_child_idx(pn.masks, pn.first, child_idx);
+        if (leaf_flag(pn.masks, child_idx)) {
+          r->hit = true;
+          r->node = child;
+          r->depth = scale_max - scale;
+          r->t = parent->parent_tmin;
+          r->loops = while_counter;
+          std::memcpy(r->center, cc, sizeof cc);
+          return;
+        }
+        if (ratio * tc_min > scale_exp2 || scale_max - scale == max_depth_cap) {
+          r->hit = true;
+          r->node = child;
+          r->depth = scale_max - scale + 1;
+          r->t = parent->parent_tmin;
+          r->loops = while_counter;
+          std::memcpy(r->center, cc, sizeof cc);
+          return;
+        }
+        parent->parent_tmin = tc_max;
+        --scale;
+        stack[scale] = StackElement{child, tc_min, tc_max, {cc[0], cc[1], cc[2]}};
+        ++r->pushes;
+        continue;
+      } else {
+        parent->parent_tmin = tc_max;
+      }
+    } else {
+      ++scale;
+      continue;
+    }
+  }
+  r->hit = false;
+  r->node = 0xFFFFFFFFu;
+  r->t = t_max;
+  r->loops = while_counter;
+}
+
+// Policy C: CpuRaycasterSingleRay3::traversSvo, scaleMax = SVO depth.
+void travers_cpu3(const Buffers& B, const float o[3], float d[3], float ratio, Result* r) {
+  r->hit = false;
+  r->loops = 0;
+  r->pushes = 0;
+  r->boundary = false;
+  r->node = 0xFFFFFFFFu;
+  r->depth = 0;
+  r->t = INFINITY;
+  r->center[0] = r->center[1] = r->center[2] = 0.0f;
+  float t_min, t_max;
+  if (!intersect_aabb(o, d, &t_min, &t_max)) return;
+  r->t = t_max;
+
+  const int scale_max = (int)std::max(1u, B.depth);
+  int scale = scale_max - 1;
+  float scale_exp2 = 0.5f;
+  const float epsilon = 0.00001f;
+  StackElement stack[kMaxStack];
+  stack[scale] = StackElement{0, t_min, t_max, {0.0f, 0.0f, 0.0f}};
+  for (int a = 0; a < 3; ++a)
+    if (std::fabs(d[a]) < epsilon) d[a] = epsilon * cl_sign(d[a]) * 10.0f;
+  const float dx_r = 1.0f / d[0], mpx = -o[0] * dx_r;
+  const float dy_r = 1.0f / d[1], mpy = -o[1] * dy_r;
+  const float dz_r = 1.0f / d[2], mpz = -o[2] * dz_r;
+  float child_size_half = 0.25f;
+  bool refetch_parent = true;
+  uint32_t parent_node_index = 0;
+  float parent_tmin = 0, parent_tmax = 0, parent_center[3] = {0, 0, 0};
+  unsigned while_counter = 0;
+  const unsigned max_loops = (unsigned)((B.depth + 1) * (B.depth + 1) * 5.0);
+
+  while (scale < scale_max) {
+    ++while_counter;
+    if (while_counter == max_loops) return;  // "MAX LOOPS REACHED": no hit
+    if (refetch_parent) {
+      parent_node_index = stack[scale].parent_node_index;
+      parent_tmin = stack[scale].parent_tmin;
+      parent_tmax = stack[scale].parent_tmax;
+      std::memcpy(parent_center, stack[scale].center, sizeof parent_center);
+      scale_exp2 = (float)std::pow(2.0, scale - scale_max);
+      child_size_half = scale_exp2 * 0.5f;
+      refetch_parent = false;
+    }
+    if (parent_tmax < 0.0) { ++scale; refetch_parent = true; continue; }
+    if (std::fabs(parent_tmin - parent_tmax) < epsilon) { ++scale; refetch_parent = true; continue; }
+    if (parent_tmin < parent_tmax) {
+      float e[3];
+      for (int a = 0; a < 3; ++a) e[a] = (o[a] + (parent_tmin + epsilon) * d[a]) - parent_center[a];
+      const int child_index = 4 * (e[0] > 0) + 2 * (e[1] > 0) + (e[2] > 0);
+      float cc[3];
+      cc[0] = (-0.5f + (bool)(child_index & 4)) * scale_exp2 + parent_center[0];
+      cc[1] = (-0.5f + (bool)(child_index & 2)) * scale_exp2 + parent_center[1];
+      cc[2] = (-0.5f + (bool)(child_index & 1)) * scale_exp2 + parent_center[2];
+      const float x0 = cc[0] - child_size_half, x1 = cc[0] + child_size_half;
+      const float y0 = cc[1] - child_size_half, y1 = cc[1] + child_size_half;
+      const float z0 = cc[2] - child_size_half, z1 = cc[2] + child_size_half;
+      float tx0 = dx_r * x0 + mpx, tx1 = dx_r * x1 + mpx;
+      swap_to_increasing(tx0, tx1);
+      float ty0 = dy_r * y0 + mpy, ty1 = dy_r * y1 + mpy;
+      swap_to_increasing(ty0, ty1);
+      float tz0 = dz_r * z0 + mpz, tz1 = dz_r * z1 + mpz;
+      swap_to_increasing(tz0, tz1);
+      const float tc_min = cl_max(tx0, cl_max(ty0, tz0));
+      const float tc_max = cl_min(tx1, cl_min(ty1, tz1));
+      {
+        const float t0[3] = {tx0, ty0, tz0}, t1[3] = {tx1, ty1, tz1};
+        if (near_boundary(e, parent_tmin, parent_tmax, epsilon, t0, t1)) r->boundary = true;
+      }
+      if (parent_node_index >= B.pool) { r->t = t_max; return; }
+      const Node pn = B.nodes[parent_node_index];
+      if (valid_flag(pn.masks, child_index)) {
+        const uint32_t child = parent_node_index + nth_child_idx(pn.masks, pn.first, child_index);
+        if (leaf_flag(pn.masks, child_index) || ratio * tc_max > scale_exp2 || scale == 0) {
+          r->hit = true;
+          r->node = child;
+          r->depth = scale_max - scale;
+          r->t = parent_tmin;
+          r->loops = while_counter;
+          std::memcpy(r->center, cc, sizeof cc);
+          return;
+        }
+        stack[scale].parent_tmin = tc_max;
+        --scale;
+        stack[scale] = StackElement{child, tc_min, tc_max, {cc[0], cc[1], cc[2]}};
+        ++r->pushes;
+        refetch_parent = true;
+        continue;
+      } else {
+        parent_tmin = tc_max;
+      }
+    } else {
+      ++scale;
+      refetch_parent = true;
+      continue;
+    }
+  }
+}
+
+}  // namespace
+
+// ------------------------------------------------------------ C interface
+
+extern "C" {
+
+int oracle_open(const char* base, oracle_scene** out) {
+  if (!base || !out) return 1;
+  std::unique_ptr<oracle_scene> s(new oracle_scene());
+  int r = read_bundle(base, s.get());
+  if (r) return r;
+  s->max_depth = bundle_depth(*s);
+  *out = s.release();
+  return 0;
+}
+
+void oracle_free(oracle_scene* s) { delete s; }
+
+int oracle_info(const oracle_scene* s, uint32_t* num_treelets, uint32_t* nodes_per_treelet, uint32_t* max_depth) {
+  if (!s) return 1;
+  if (num_treelets) *num_treelets = (uint32_t)s->treelets.size();
+  if (nodes_per_treelet) *nodes_per_treelet = s->N;
+  if (max_depth) *max_depth = s->max_depth;
+  return 0;
+}
+
+int oracle_make_incore(oracle_scene* s, uint64_t incore_bytes) { return s ? make_incore(s, incore_bytes) : 1; }
+
+// Borrowed pointers to the stitched incore buffer: nodes (2 words each),
+// colours, normals, slot table.
+int oracle_incore(const oracle_scene* s, const uint32_t** nodes, const uint32_t** colors, const uint32_t** normals,
+                  const int32_t** slot_gid, uint64_t* pool_nodes, uint32_t* num_slots) {
+  if (!s || s->incore.empty()) return 6;
+  if (nodes) *nodes = reinterpret_cast<const uint32_t*>(s->incore.data());
+  if (colors) *colors = s->colors.data();
+  if (normals) *normals = s->normals.data();
+  if (slot_gid) *slot_gid = s->slot_gid.data();
+  if (pool_nodes) *pool_nodes = s->incore.size();
+  if (num_slots) *num_slots = s->num_slots;
+  return 0;
+}
+
+// Renders one frame over arbitrary incore buffers.  Any output may be null.
+//   t, node, treelet, rgba : as the library's svo_frame_out
+//   boundary               : 1 if the ray is a boundary ray
+//   pushes                 : number of stack pushes of the ray
+//   depth, loops           : SampleResult depth / numWhileLoops
+//   center                 : 3 floats, nodeCenter of the hit
+int oracle_render_buffers(const uint32_t* nodes, uint64_t pool_nodes, const uint32_t* colors, const int32_t* slot_gid,
+                          uint32_t nodes_per_treelet, uint32_t num_slots, uint32_t depth, const void* camera,
+                          int policy, float* t, uint32_t* node, uint32_t* treelet, uint32_t* rgba, uint8_t* boundary,
+                          uint32_t* pushes, uint32_t* depth_out, uint32_t* loops, float* center, int threads) {
+  if (!nodes || !colors || !slot_gid || !camera || nodes_per_treelet == 0) return 1;
+  if (policy != 0 && policy != 2) return 1;
+  Camera cam;
+  std::memcpy(&cam, camera, sizeof cam);
+  const Buffers B{reinterpret_cast<const Node*>(nodes), colors, slot_gid, pool_nodes, nodes_per_treelet, num_slots, depth};
+  if (std::max(13u, depth + 1) >= (uint32_t)kMaxStack) return 1;
+  const int W = (int)cam.width, H = (int)cam.height;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads > 0 ? threads : 1)
+  for (int y = 0; y < H; ++y) {
+    for (int x = 0; x < W; ++x) {
+      float d[3];
+      pixel_ray(cam, (uint32_t)x, (uint32_t)y, d);
+      const float o[3] = {cam.position[0], cam.position[1], cam.position[2]};
+      Result r;
+      if (policy == 0) sample_treelet(B, o, d, cam.t_scale_ratio, &r);
+      else travers_cpu3(B, o, d, cam.t_scale_ratio, &r);
+      const size_t p = (size_t)y * W + x;
+      uint32_t tr = 0xFFFFFFFFu, col = 0x334D33FFu;
+      if (r.hit && r.node < pool_nodes) {
+        tr = (uint32_t)slot_gid[r.node / nodes_per_treelet];
+        col = colors[r.node] | 0xFFu;
+      }
+      if (t) t[p] = r.t;
+      if (node) node[p] = r.hit ? r.node : 0xFFFFFFFFu;
+      if (treelet) treelet[p] = tr;
+      if (rgba) rgba[p] = col;
+      if (boundary) boundary[p] = r.boundary ? 1 : 0;
+      if (pushes) pushes[p] = r.pushes;
+      if (depth_out) depth_out[p] = r.depth;
+      if (loops) loops[p] = r.loops;
+      if (center) { center[3 * p] = r.center[0]; center[3 * p + 1] = r.center[1]; center[3 * p + 2] = r.center[2]; }
+    }
+  }
+  return 0;
+}
+
+int oracle_render(const oracle_scene* s, const void* camera, int policy, float* t, uint32_t* node, uint32_t* treelet,
+                  uint32_t* rgba, uint8_t* boundary, uint32_t* pushes, uint32_t* depth_out, uint32_t* loops,
+                  float* center, int threads) {
+  if (!s || s->incore.empty()) return 6;
+  return oracle_render_buffers(reinterpret_cast<const uint32_t*>(s->incore.data()), s->incore.size(), s->colors.data(),
+                               s->slot_gid.data(), s->N, s->num_slots, s->max_depth, camera, policy, t, node, treelet,
+                               rgba, boundary, pushes, depth_out, loops, center, threads);
+}
+
+// One ray with explicit origin / direction (direction used as given).
+int oracle_trace_ray(const oracle_scene* s, const float origin[3], const float direction[3], float ratio, int policy,
+                     uint32_t* node, float* t, uint32_t* depth_out, float center[3], uint8_t* boundary) {
+  if (!s || s->incore.empty() || !origin || !direction) return 1;
+  const Buffers B{s->incore.data(), s->colors.data(), s->slot_gid.data(), s->incore.size(), s->N, s->num_slots,
+                  s->max_depth};
+  float d[3] = {direction[0], direction[1], direction[2]};
+  Result r;
+  if (policy == 0) sample_treelet(B, origin, d, ratio, &r);
+  else travers_cpu3(B, origin, d, ratio, &r);
+  if (node) *node = r.hit ? r.node : 0xFFFFFFFFu;
+  if (t) *t = r.t;
+  if (depth_out) *depth_out = r.depth;
+  if (center) std::memcpy(center, r.center, sizeof r.center);
+  if (boundary) *boundary = r.boundary;
+  return 0;
+}
+
+// Ray directions of every pixel (3 floats per pixel, row-major).
+int oracle_ray_dirs(const void* camera, float* dirs) {
+  if (!camera || !dirs) return 1;
+  Camera cam;
+  std::memcpy(&cam, camera, sizeof cam);
+  for (uint32_t y = 0; y < cam.height; ++y)
+    for (uint32_t x = 0; x < cam.width; ++x) pixel_ray(cam, x, y, dirs + 3 * ((size_t)y * cam.width + x));
+  return 0;
+}
+
+int oracle_max_threads(void) { return omp_get_max_threads(); }
+
+}  // extern "C"
