@@ -75,6 +75,9 @@ def main():
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="after the timed launches, run one plain device copy of this many MiB "
+                         "(known traffic for scaling FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the single-GPU build has no "
@@ -119,6 +122,15 @@ def main():
     achieved = alg_bytes / (ms / 1e3) / 1e9
     log(f"{args.steps} launches: {ms:.4f} ms/launch (events), wall {wall * 1e3 / args.steps:.4f} ms/launch, "
         f"{value:.1f} Mrays/s, {achieved:.1f} GB/s algorithmic")
+
+    if args.copy_calibration_mb > 0:
+        nbytes = args.copy_calibration_mb << 20
+        src, dst = svo.DeviceBuffer(nbytes), svo.DeviceBuffer(nbytes)
+        src.fill(1)
+        svo._capi.check(svo.lib.svo_device_synchronize(), "sync")
+        svo._capi.check(svo.lib.svo_debug_copy(dst.ptr, src.ptr, nbytes, stream.handle), "svo_debug_copy")
+        stream.synchronize()
+        log(f"calibration copy of {nbytes} bytes done")
 
     out = {
         "metric": METRIC,

@@ -323,6 +323,12 @@ svo_status svo_render_timed(svo_mm* mm, const svo_camera* cams,
 svo_status svo_debug_ray_dirs(const svo_camera* cam, float* dirs_dev,
                               void* stream);
 
+/* Diagnostic: device-to-device copy of `bytes` (a multiple of 16) with a
+ * plain streaming kernel; a dispatch of known traffic for calibrating the
+ * FETCH_SIZE / WRITE_SIZE hardware counters. */
+svo_status svo_debug_copy(void* dst, const void* src, size_t bytes,
+                          void* stream);
+
 #ifdef __cplusplus
 }
 #endif

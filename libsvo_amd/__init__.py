@@ -134,6 +134,18 @@ class Scene:
         attr = np.ctypeslib.as_array(ctypes.cast(pa, ctypes.POINTER(ctypes.c_uint32)), (n, 2)).copy()
         return {k: getattr(h, k) for k, _ in _capi.svo_treelet_header._fields_}, nodes, attr
 
+    def headers(self) -> np.ndarray:
+        """(num_treelets, 8) uint32 array of every treelet header, in file order
+        (num_nodes, num_leaves, mem_size, gid, parent_gid, parent_leaf_pos,
+        parent_leaf_idx, parent_leaf_parent_pos)."""
+        n = self.info["num_treelets"]
+        out = np.empty((n, 8), np.uint32)
+        h = _capi.svo_treelet_header()
+        for g in range(n):
+            check(lib.svo_scene_get_treelet(self._h, g, ctypes.byref(h), None, None), "svo_scene_get_treelet")
+            out[g] = [getattr(h, k) for k, _ in _capi.svo_treelet_header._fields_]
+        return out
+
     def getTreeletSizeInByte(self) -> int:
         return self.info["treelet_bytes"]
 

@@ -99,6 +99,7 @@ SIGNATURES = {
     "svo_device_synchronize": (ctypes.c_int, []),
     "svo_render_timed": (ctypes.c_int, [vp, P(svo_camera), u32, P(svo_frame_out), ctypes.c_int, vp, u32, P(f32)]),
     "svo_debug_ray_dirs": (ctypes.c_int, [P(svo_camera), vp, vp]),
+    "svo_debug_copy": (ctypes.c_int, [vp, vp, ctypes.c_size_t, vp]),
 }
 
 

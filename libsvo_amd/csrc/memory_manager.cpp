@@ -574,3 +574,10 @@ extern "C" svo_status svo_debug_ray_dirs(const svo_camera* cam, float* dirs_dev,
   if (e != 0) { set_error("ray-direction launch failed: %s", hipGetErrorString((hipError_t)e)); return SVO_ERR_HIP; }
   return SVO_OK;
 }
+
+extern "C" svo_status svo_debug_copy(void* dst, const void* src, size_t bytes, void* stream) {
+  if (!dst || !src || (bytes % 16) != 0) { set_error("svo_debug_copy: null pointer or size not a multiple of 16"); return SVO_ERR_ARG; }
+  int e = launch_copy(dst, src, bytes, stream);
+  if (e != 0) { set_error("copy launch failed: %s", hipGetErrorString((hipError_t)e)); return SVO_ERR_HIP; }
+  return SVO_OK;
+}

@@ -216,37 +216,65 @@ svo_status build_scene(uint32_t depth, std::vector<uint64_t>& codes,
   const uint32_t N = tb / 8;
 
   // Dense packing: cut[d][i] marks the roots of sub-treelets.  Bottom-up,
-  // each node carries the size of the part of its subtree that stays in its
-  // treelet (a cut child costs one anchor node); while that exceeds N the
-  // heaviest remaining child is cut (greedy minimum-count partition of a tree
-  // into connected parts of bounded size).
+  // each node carries w = the size of the part of its subtree that stays in
+  // its treelet (a cut node costs one anchor node); while a node's w exceeds
+  // N the heaviest cuttable nodes below it are cut (greedy partition of a
+  // tree into connected parts of bounded size).
+  // Cuts are made only at even levels >= 2.  The anchor of a sub-treelet is
+  // therefore never a child of a treelet root: the reference's stitch step
+  // clears the anchor's leaf bit in its parent inside the parent treelet's
+  // slot (src/TreeletMemoryManager.cpp:534-535), which would miss the copy of
+  // a treelet root stitched into the level above.  A node whose children sit
+  // at an odd level cuts among its grandchildren instead.
   std::vector<std::vector<uint8_t>> cut(depth + 1);
   if (!bfs) {
-    std::vector<uint32_t> w_child(lv[depth].attr.size() / 2, 1u), w_cur;
-    cut[depth].assign(w_child.size(), 0);
+    if (N < 80) { set_error("dense packing needs treelets of at least 80 nodes (got %u)", N); return SVO_ERR_ARG; }
+    std::vector<std::vector<uint32_t>> w(depth + 1);
+    w[depth].assign(lv[depth].attr.size() / 2, 1u);
+    cut[depth].assign(w[depth].size(), 0);
+    struct Cand { uint32_t w; uint32_t level; uint32_t index; uint32_t parent_slot; };
     for (int d = (int)depth - 1; d >= 0; --d) {
       const Level& P = lv[d];
       const size_t np = P.mask.size();
-      w_cur.assign(np, 0);
+      w[d].assign(np, 0);
       cut[d].assign(np, 0);
+      const bool child_even = ((d + 1) % 2) == 0 && d + 1 >= 2;
       for (size_t p = 0; p < np; ++p) {
         const uint32_t f = P.first[p];
         const uint32_t nc = (uint32_t)__builtin_popcount(P.mask[p]);
         uint64_t sum = 1;
-        uint32_t order[8];
-        for (uint32_t k = 0; k < nc; ++k) { sum += w_child[f + k]; order[k] = k; }
+        for (uint32_t k = 0; k < nc; ++k) sum += w[d + 1][f + k];
         if (sum > N) {
-          std::sort(order, order + nc, [&](uint32_t a, uint32_t b) {
-            return w_child[f + a] != w_child[f + b] ? w_child[f + a] > w_child[f + b] : a < b;
-          });
-          for (uint32_t k = 0; k < nc && sum > N; ++k) {
-            cut[d + 1][f + order[k]] = 1;
-            sum -= (uint64_t)w_child[f + order[k]] - 1;
+          Cand cand[64];
+          uint32_t n = 0;
+          if (child_even) {
+            for (uint32_t k = 0; k < nc; ++k)
+              if ((uint32_t)(d + 1) < depth) cand[n++] = Cand{w[d + 1][f + k], (uint32_t)d + 1, f + k, k};
+          } else if ((uint32_t)(d + 2) <= depth) {
+            for (uint32_t k = 0; k < nc; ++k) {
+              const uint32_t c = f + k;
+              if ((uint32_t)(d + 1) >= depth) continue;
+              const uint32_t gf = lv[d + 1].first[c];
+              const uint32_t gn = (uint32_t)__builtin_popcount(lv[d + 1].mask[c]);
+              for (uint32_t q = 0; q < gn; ++q)
+                if ((uint32_t)(d + 2) < depth && !cut[d + 2][gf + q]) cand[n++] = Cand{w[d + 2][gf + q], (uint32_t)d + 2, gf + q, c};
+            }
           }
+          std::sort(cand, cand + n, [](const Cand& a, const Cand& b) {
+            return a.w != b.w ? a.w > b.w : (a.level != b.level ? a.level < b.level : a.index < b.index);
+          });
+          for (uint32_t k = 0; k < n && sum > N; ++k) {
+            if (cand[k].w <= 1) break;
+            cut[cand[k].level][cand[k].index] = 1;
+            const uint64_t dec = (uint64_t)cand[k].w - 1;
+            sum -= dec;
+            if (cand[k].level == (uint32_t)d + 2) w[d + 1][cand[k].parent_slot] -= (uint32_t)dec;
+          }
+          if (sum > N) { set_error("dense packing: node at level %d keeps %llu > %u nodes", d, (unsigned long long)sum, N); return SVO_ERR_ARG; }
         }
-        w_cur[p] = (uint32_t)sum;
+        w[d][p] = (uint32_t)sum;
       }
-      w_child.swap(w_cur);
+      if (d + 2 <= (int)depth) std::vector<uint32_t>().swap(w[d + 2]);
     }
   }
 

@@ -42,4 +42,7 @@ int launch_trace(const TracePools& pools, const TraceFrame* frames_dev, uint32_t
 // (3 floats per pixel, row-major) -- used to pin the ray generation.
 int launch_ray_dirs(const svo_camera& cam, float* dirs_dev, void* stream);
 
+// Copies bytes/16 16-byte words with a plain streaming kernel.
+int launch_copy(void* dst, const void* src, size_t bytes, void* stream);
+
 }  // namespace svo_impl

@@ -21,6 +21,8 @@
 // restatement bit for bit.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "trace.h"
 
 namespace svo_impl {
@@ -280,7 +282,21 @@ __global__ void ray_dir_kernel(svo_camera cam, float* dirs) {
   dirs[3 * p + 2] = d[2];
 }
 
+// Plain streaming copy, 16 bytes per lane per iteration (counter calibration).
+__global__ __launch_bounds__(256) void copy_kernel(uint4* __restrict__ dst, const uint4* __restrict__ src, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 }  // namespace
+
+int launch_copy(void* dst, const void* src, size_t bytes, void* stream) {
+  const size_t n = bytes / 16;
+  if (n == 0) return 0;
+  const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 256 * 64);
+  hipLaunchKernelGGL(copy_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream, (uint4*)dst, (const uint4*)src, n);
+  return (int)hipGetLastError();
+}
 
 int launch_trace(const TracePools& pools, const TraceFrame* frames_dev, uint32_t num_frames, uint32_t num_tiles,
                  int policy, void* stream) {
