@@ -27,8 +27,7 @@
 //   get_color        treeletRenderer_rgba.cl:183-195,399-404,645-655
 //
 // Pinning status of each function is recorded in tests/test_oracle_cpu.py;
-// the boundary-ray flag is this project's definition (a ray is flagged when a
-// child-selection step comes within 1e-6 of a face or of the pop epsilon).
+// the boundary-ray flag is this project's definition (see kBoundary below).
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -240,22 +239,38 @@ void pixel_ray(const Camera& c, uint32_t x, uint32_t y, float d[3]) {
   for (int a = 0; a < 3; ++a) d[a] = v[a] / len;
 }
 
-// Boundary test of one child-selection step (this project's definition).
-inline bool near_boundary(const float e[3], float tmin, float tmax, float eps, const float t0[3], const float t1[3]) {
-  const float b = 1e-6f;
-  if (std::fabs(e[0]) < b || std::fabs(e[1]) < b || std::fabs(e[2]) < b) return true;
-  if (std::fabs(std::fabs(tmin - tmax) - eps) < b) return true;
-  // two largest entry planes / two smallest exit planes
+// Boundary rays (this project's definition; the reference has none).  A ray
+// is flagged when its result can depend on rounding or on the policy:
+//   * a direction component below 1e-4 (the policies clamp differently);
+//   * a step enters or leaves a child within 1e-6 (in t) of an edge or corner
+//     (the two largest entry-plane t, or the two smallest exit-plane t, tie),
+//     or clips a child over less than 1e-6;
+//   * a pop test is within 1e-6 of the epsilon threshold, or sees an inverted
+//     interval tmin > tmax + epsilon (policy A processes it, policy C pops);
+//   * the ray exhausts maxLoops.
+constexpr float kBoundary = 1e-6f;
+
+inline bool near_edge(const float t0[3], const float t1[3], float tc_min, float tc_max) {
+  if (tc_max - tc_min < kBoundary) return true;
   float a0 = t0[0], a1 = t0[1], a2 = t0[2];
   if (a0 < a1) std::swap(a0, a1);
   if (a1 < a2) std::swap(a1, a2);
   if (a0 < a1) std::swap(a0, a1);
-  if (std::fabs(a0 - a1) < b) return true;
+  if (std::fabs(a0 - a1) < kBoundary) return true;
   float c0 = t1[0], c1 = t1[1], c2 = t1[2];
   if (c0 > c1) std::swap(c0, c1);
   if (c1 > c2) std::swap(c1, c2);
   if (c0 > c1) std::swap(c0, c1);
-  return std::fabs(c0 - c1) < b;
+  return std::fabs(c0 - c1) < kBoundary;
+}
+
+inline bool near_pop_threshold(float tmin, float tmax, float eps) {
+  const float diff = std::fabs(tmin - tmax);
+  return std::fabs(diff - eps) < kBoundary || (tmin > tmax && diff > eps);
+}
+
+inline bool small_component(const float d[3]) {
+  return std::fabs(d[0]) < 1e-4f || std::fabs(d[1]) < 1e-4f || std::fabs(d[2]) < 1e-4f;
 }
 
 struct StackElement {
@@ -288,6 +303,7 @@ void sample_treelet(const Buffers& B, const float o[3], float d[3], float ratio,
   const float epsilon = 0.00001f;
   StackElement stack[kMaxStack];
   stack[scale] = StackElement{0, t_min, t_max, {0.0f, 0.0f, 0.0f}};
+  if (small_component(d)) r->boundary = true;
   for (int a = 0; a < 3; ++a)
     if (std::fabs(d[a]) < min_normal) d[a] = min_normal * cl_sign(d[a]);
   const float dx_r = 1.0f / d[0], mpx = -o[0] * dx_r;
@@ -302,6 +318,7 @@ void sample_treelet(const Buffers& B, const float o[3], float d[3], float ratio,
     const float child_size_half = scale_exp2 * 0.5f;
     ++while_counter;
     if (while_counter == max_loops) {
+      r->boundary = true;
       r->hit = true;
       r->node = parent->parent_node_index;
       r->depth = scale_max - scale;
@@ -310,6 +327,7 @@ void sample_treelet(const Buffers& B, const float o[3], float d[3], float ratio,
       std::memcpy(r->center, parent->center, sizeof r->center);
       return;
     }
+    if (near_pop_threshold(parent->parent_tmin, parent->parent_tmax, epsilon)) r->boundary = true;
     if (parent->parent_tmax < 0.0f) { ++scale; continue; }
     if (std::fabs(parent->parent_tmin - parent->parent_tmax) > epsilon) {
       float e[3];
@@ -332,7 +350,7 @@ void sample_treelet(const Buffers& B, const float o[3], float d[3], float ratio,
       const float tc_max = cl_min(tx1, cl_min(ty1, tz1));
       {
         const float t0[3] = {tx0, ty0, tz0}, t1[3] = {tx1, ty1, tz1};
-        if (near_boundary(e, parent->parent_tmin, parent->parent_tmax, epsilon, t0, t1)) r->boundary = true;
+        if (near_edge(t0, t1, tc_min, tc_max)) r->boundary = true;
       }
       const uint64_t pidx = parent->parent_node_index;
       if (pidx >= B.pool) return;  // corrupt index: miss
@@ -396,6 +414,7 @@ void travers_cpu3(const Buffers& B, const float o[3], float d[3], float ratio, R
   const float epsilon = 0.00001f;
   StackElement stack[kMaxStack];
   stack[scale] = StackElement{0, t_min, t_max, {0.0f, 0.0f, 0.0f}};
+  if (small_component(d)) r->boundary = true;
   for (int a = 0; a < 3; ++a)
     if (std::fabs(d[a]) < epsilon) d[a] = epsilon * cl_sign(d[a]) * 10.0f;
   const float dx_r = 1.0f / d[0], mpx = -o[0] * dx_r;
@@ -410,7 +429,10 @@ void travers_cpu3(const Buffers& B, const float o[3], float d[3], float ratio, R
 
   while (scale < scale_max) {
     ++while_counter;
-    if (while_counter == max_loops) return;  // "MAX LOOPS REACHED": no hit
+    if (while_counter == max_loops) {  // "MAX LOOPS REACHED": no hit
+      r->boundary = true;
+      return;
+    }
     if (refetch_parent) {
       parent_node_index = stack[scale].parent_node_index;
       parent_tmin = stack[scale].parent_tmin;
@@ -420,6 +442,7 @@ void travers_cpu3(const Buffers& B, const float o[3], float d[3], float ratio, R
       child_size_half = scale_exp2 * 0.5f;
       refetch_parent = false;
     }
+    if (near_pop_threshold(parent_tmin, parent_tmax, epsilon)) r->boundary = true;
     if (parent_tmax < 0.0) { ++scale; refetch_parent = true; continue; }
     if (std::fabs(parent_tmin - parent_tmax) < epsilon) { ++scale; refetch_parent = true; continue; }
     if (parent_tmin < parent_tmax) {
@@ -443,7 +466,7 @@ void travers_cpu3(const Buffers& B, const float o[3], float d[3], float ratio, R
       const float tc_max = cl_min(tx1, cl_min(ty1, tz1));
       {
         const float t0[3] = {tx0, ty0, tz0}, t1[3] = {tx1, ty1, tz1};
-        if (near_boundary(e, parent_tmin, parent_tmax, epsilon, t0, t1)) r->boundary = true;
+        if (near_edge(t0, t1, tc_min, tc_max)) r->boundary = true;
       }
       if (parent_node_index >= B.pool) { r->t = t_max; return; }
       const Node pn = B.nodes[parent_node_index];
