@@ -447,6 +447,10 @@ svo_status make_pools(const svo_mm* mm, int policy, TracePools* P) {
     return SVO_ERR_ARG;
   }
   if (P->scale_max > 16) { set_error("scene depth %u exceeds the 16-level traversal stack", D); return SVO_ERR_ARG; }
+  // a push leaves scale s for s-1; policy A never pushes from scale
+  // scale_max - lod_cap (the depth cap stops there), policy C never from 0
+  P->min_push_scale = P->lod_cap != 0 ? P->scale_max - P->lod_cap + 1 : 1;
+  P->stack_levels = P->scale_max > P->min_push_scale ? P->scale_max - P->min_push_scale : 1;
   return SVO_OK;
 }
 

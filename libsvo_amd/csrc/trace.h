@@ -21,6 +21,8 @@ struct TracePools {
   uint32_t scale_max;     // policy scaleMax
   uint32_t max_loops;     // policy maxLoops
   uint32_t lod_cap;       // policy-A depth cap (MAX_SVO_RAYCAST_DEPTH), 0 = none
+  uint32_t min_push_scale;  // lowest scale a push can start from
+  uint32_t stack_levels;    // scale_max - min_push_scale (LDS stack depth)
 };
 
 // One camera of a launch: its uniforms, its outputs and its first tile.

@@ -2,9 +2,9 @@
 //
 // One ray per lane, one 8x8 pixel tile per wavefront, four wavefronts per
 // workgroup.  The per-ray stack lives in LDS ([level][lane], conflict-free),
-// holding only what cannot be recomputed:
-//   * the parent's child base (node index + first-child offset) and masks;
-//   * the child index taken from that parent (3 bits, packed above the masks).
+// 4 bytes per level: the node index of each parent on the path.  The child
+// index taken from each parent goes to a 64-bit path register (3 bits per
+// level); the parent node itself is re-read on a pop.
 // A parent's centre, its tmax and its tmin after the pop are recomputed
 // bit-exactly instead of stored:
 //   * centres are dyadic, so parent = child - (-0.5 + bit) * 2^(s - scaleMax)
@@ -100,9 +100,19 @@ __device__ __forceinline__ float node_tmax(const float c[3], float h, const floa
 // POLICY 0: treeletRenderer_rgba.cl::sample (:201-382).
 // POLICY 2: CpuRaycasterSingleRay3::traversSvo
 //           (src/incore/CpuRaycasterSingleRay3.cpp:158-387).
+//
+// Stack: s_idx[lvl][lane] holds the node index of the parent left by a push,
+// lvl = scale - min_push_scale; the child index taken at each push is kept in
+// `path`, 3 bits per level.  A pop re-reads the parent node (a cache hit: it
+// was fetched a few iterations earlier).  Every node value therefore comes
+// from the one conditional load at the end of an iteration, and its first
+// use is the mask test of the next iteration, after the entry-point and plane
+// arithmetic that overlaps the fetch.  (Running a lane's pops back to back
+// before its next child step was measured slower: it serialises the pop
+// chains of a wave.)
 template <int POLICY>
-__device__ Hit traverse(const TracePools& S, const float o[3], float d[3], float ratio, uint32_t* s_base,
-                        uint32_t* s_mask, uint32_t tid) {
+__device__ Hit traverse(const TracePools& S, const float o[3], float d[3], float ratio, uint32_t* s_idx,
+                        uint32_t tid) {
   Hit r{0xFFFFFFFFu, __int_as_float(0x7F800000), 0u};
   float t_min, t_max;
   if (!root_box(o, d, t_min, t_max)) return r;
@@ -125,12 +135,13 @@ __device__ Hit traverse(const TracePools& S, const float o[3], float d[3], float
   }
 
   const int smax = (int)S.scale_max;
+  const int mps = (int)S.min_push_scale;
   int scale = smax - 1;
   float c[3] = {0.0f, 0.0f, 0.0f};
   float tmin = t_min, tmax = t_max;
-  const uint64_t root = S.nodes[0];
-  uint32_t base = (uint32_t)root;  // node index 0 + first child
-  uint32_t masks = (uint32_t)(root >> 32);
+  uint32_t nidx = 0;  // index of the current node
+  uint64_t nd = S.nodes[0];
+  uint64_t path = 0;
   uint32_t iter = 0;
 
   for (;;) {
@@ -139,14 +150,7 @@ __device__ Hit traverse(const TracePools& S, const float o[3], float d[3], float
     ++iter;
     if (iter == S.max_loops) {
       if (POLICY == 0) {  // maxLoops returns the current parent as a hit
-        uint32_t own = 0;
-        if (scale + 1 < smax) {
-          const uint32_t b = s_base[(scale + 1) * kBlock + tid];
-          const uint32_t m = s_mask[(scale + 1) * kBlock + tid];
-          const uint32_t cc = m >> 16;
-          own = b + (uint32_t)__popc((m >> 8) & ((1u << cc) - 1u));
-        }
-        r.node = own;
+        r.node = nidx;
         r.t = tmin;
       }
       return r;
@@ -155,6 +159,8 @@ __device__ Hit traverse(const TracePools& S, const float o[3], float d[3], float
     if (POLICY == 0) pop = (tmax < 0.0f) || !(fabsf(tmin - tmax) > eps);
     else pop = (tmax < 0.0f) || (fabsf(tmin - tmax) < eps) || !(tmin < tmax);
 
+    bool fetch = false;
+    uint32_t fidx = 0;
     if (!pop) {
       const float te = tmin + eps;
       const float ex = (o[0] + te * d[0]) - c[0];
@@ -177,8 +183,9 @@ __device__ Hit traverse(const TracePools& S, const float o[3], float d[3], float
       const float tc_min = cl_max(t0[0], cl_max(t0[1], t0[2]));
       const float tc_max = cl_min(t1[0], cl_min(t1[1], t1[2]));
 
+      const uint32_t masks = (uint32_t)(nd >> 32);
       if ((masks >> (8u + ci)) & 1u) {
-        const uint32_t child = base + (uint32_t)__popc((masks >> 8) & ((1u << ci) - 1u));
+        const uint32_t child = nidx + (uint32_t)nd + (uint32_t)__popc((masks >> 8) & ((1u << ci) - 1u));
         if ((masks >> ci) & 1u) {  // leaf
           r.node = child;
           r.t = tmin;
@@ -198,15 +205,14 @@ __device__ Hit traverse(const TracePools& S, const float o[3], float d[3], float
         }
         // push
         ++r.pushes;
-        s_base[scale * kBlock + tid] = base;
-        s_mask[scale * kBlock + tid] = (masks & 0xFFFFu) | (ci << 16);
+        s_idx[(scale - mps) * kBlock + tid] = nidx;
+        path = (path << 3) | ci;
         --scale;
         c[0] = cc[0]; c[1] = cc[1]; c[2] = cc[2];
         tmin = tc_min;
         tmax = tc_max;
-        const uint64_t nd = S.nodes[child];
-        base = child + (uint32_t)nd;
-        masks = (uint32_t)(nd >> 32);
+        fidx = child;
+        fetch = true;
       } else {
         tmin = tc_max;  // advance
       }
@@ -214,10 +220,10 @@ __device__ Hit traverse(const TracePools& S, const float o[3], float d[3], float
       ++scale;
       if (scale >= smax) return r;  // left the root: miss, t = root tMax
       const float t_child = tmax;
-      base = s_base[scale * kBlock + tid];
-      const uint32_t m = s_mask[scale * kBlock + tid];
-      masks = m & 0xFFFFu;
-      const uint32_t ci = m >> 16;
+      fidx = s_idx[(scale - mps) * kBlock + tid];
+      fetch = true;
+      const uint32_t ci = (uint32_t)(path & 7u);
+      path >>= 3;
       const float sp = exp2i(scale - smax);
       c[0] = c[0] - (-0.5f + (float)((ci >> 2) & 1u)) * sp;
       c[1] = c[1] - (-0.5f + (float)((ci >> 1) & 1u)) * sp;
@@ -225,14 +231,17 @@ __device__ Hit traverse(const TracePools& S, const float o[3], float d[3], float
       tmin = t_child;
       tmax = (scale == smax - 1) ? t_max : node_tmax(c, sp, rc, mo);
     }
+    if (fetch) {
+      nidx = fidx;
+      nd = S.nodes[fidx];
+    }
   }
 }
 
 template <int POLICY>
 __global__ __launch_bounds__(kBlock) void trace_kernel(TracePools S, const TraceFrame* __restrict__ frames,
                                                        uint32_t num_frames, uint32_t num_tiles) {
-  __shared__ uint32_t s_base[kMaxScale * kBlock];
-  __shared__ uint32_t s_mask[kMaxScale * kBlock];
+  extern __shared__ uint32_t s_idx[];
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = tid & 63u;
   const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -254,7 +263,7 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(TracePools S, const Trace
   const float o[3] = {F.cam.position[0], F.cam.position[1], F.cam.position[2]};
   float d[3];
   pixel_ray(F.cam, x, y, d);
-  const Hit hit = traverse<POLICY>(S, o, d, F.cam.t_scale_ratio, s_base, s_mask, tid);
+  const Hit hit = traverse<POLICY>(S, o, d, F.cam.t_scale_ratio, s_idx, tid);
 
   const size_t pix = (size_t)y * F.cam.width + x;
   uint32_t tree = 0xFFFFFFFFu, rgba = 0x334D33FFu;
@@ -301,12 +310,14 @@ int launch_copy(void* dst, const void* src, size_t bytes, void* stream) {
 int launch_trace(const TracePools& pools, const TraceFrame* frames_dev, uint32_t num_frames, uint32_t num_tiles,
                  int policy, void* stream) {
   if (num_tiles == 0) return 0;
+  if (pools.stack_levels > kMaxScale) return (int)hipErrorInvalidValue;
   const dim3 grid((num_tiles + kWaves - 1) / kWaves), block(kBlock);
+  const size_t lds = (size_t)pools.stack_levels * kBlock * sizeof(uint32_t);
   hipStream_t st = (hipStream_t)stream;
   if (policy == SVO_POLICY_CPU3)
-    hipLaunchKernelGGL(trace_kernel<2>, grid, block, 0, st, pools, frames_dev, num_frames, num_tiles);
+    hipLaunchKernelGGL(trace_kernel<2>, grid, block, lds, st, pools, frames_dev, num_frames, num_tiles);
   else
-    hipLaunchKernelGGL(trace_kernel<0>, grid, block, 0, st, pools, frames_dev, num_frames, num_tiles);
+    hipLaunchKernelGGL(trace_kernel<0>, grid, block, lds, st, pools, frames_dev, num_frames, num_tiles);
   return (int)hipGetLastError();
 }
 
