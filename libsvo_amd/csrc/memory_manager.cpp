@@ -481,9 +481,9 @@ svo_status stage_frames(svo_mm* mm, const svo_camera* cams, uint32_t n, const sv
     F.cam = c;
     F.out = outs[i];
     F.tile_begin = (uint32_t)tiles;
-    F.tiles_x = (c.width + 7) / 8;
+    F.tiles_x = (c.width + kTileW - 1) / kTileW;
     F.pad[0] = F.pad[1] = 0;
-    tiles += (uint64_t)F.tiles_x * ((c.height + 7) / 8);
+    tiles += (uint64_t)F.tiles_x * ((c.height + kTileH - 1) / kTileH);
   }
   if (tiles > 0x7FFFFFFFull) { set_error("launch of %llu tiles too large", (unsigned long long)tiles); return SVO_ERR_ARG; }
   HIP_TRY(hipMemcpyAsync(mm->d_frames, mm->h_frames, n * sizeof(TraceFrame), hipMemcpyHostToDevice, st));

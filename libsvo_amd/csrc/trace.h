@@ -25,16 +25,20 @@ struct TracePools {
   uint32_t stack_levels;    // scale_max - min_push_scale (LDS stack depth)
 };
 
+// Pixel tile of one wavefront (64 lanes).
+constexpr uint32_t kTileW = 16;
+constexpr uint32_t kTileH = 4;
+
 // One camera of a launch: its uniforms, its outputs and its first tile.
 struct TraceFrame {
   svo_camera cam;
   svo_frame_out out;
   uint32_t tile_begin;    // first global tile id of this frame
-  uint32_t tiles_x;       // tiles per row (8x8 pixels per tile)
+  uint32_t tiles_x;       // tiles per row (kTileW x kTileH pixels per tile)
   uint32_t pad[2];
 };
 
-// Enqueues the traversal of `num_tiles` 8x8 tiles spread over `num_frames`
+// Enqueues the traversal of `num_tiles` tiles spread over `num_frames`
 // frames (frames[] in device memory, sorted by tile_begin) on `stream`.
 // Returns a hipError_t value (0 = success).
 int launch_trace(const TracePools& pools, const TraceFrame* frames_dev, uint32_t num_frames,

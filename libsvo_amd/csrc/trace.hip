@@ -1,6 +1,6 @@
 // Serialized-SVO treelet traversal for gfx950.
 //
-// One ray per lane, one 8x8 pixel tile per wavefront, four wavefronts per
+// One ray per lane, one 16x4 pixel tile per wavefront, four wavefronts per
 // workgroup.  The per-ray stack lives in LDS ([level][lane], conflict-free),
 // 4 bytes per level: the node index of each parent on the path.  The child
 // index taken from each parent goes to a 64-bit path register (3 bits per
@@ -31,6 +31,7 @@ namespace {
 constexpr int kWaves = 4;
 constexpr int kBlock = 64 * kWaves;
 constexpr int kMaxScale = 16;  // stack levels; scaleMax <= 16
+static_assert(kTileW * kTileH == 64, "one wavefront per tile");
 
 // OpenCL max/min as specified for floats: max -> y if x < y else x,
 // min -> y if y < x else x.
@@ -143,11 +144,38 @@ __device__ Hit traverse(const TracePools& S, const float o[3], float d[3], float
   uint64_t nd = S.nodes[0];
   uint64_t path = 0;
   uint32_t iter = 0;
+  // Cycle detection.  The traversal is deterministic and its whole state is
+  // (scale, path, tmin, tmax): the path fixes the node and its ancestors, and
+  // every other quantity is recomputed from them.  Some rays fall into an
+  // exact cycle (an advance that leaves tmin unchanged, or a push/pop loop
+  // around a child that is re-entered at the same t) and spin until maxLoops,
+  // which in the reference returns the parent as a hit (policy A) or a miss
+  // (policy C).  A snapshot taken at every power-of-two iteration >= 128 is
+  // compared with later states; on a repeat with period p the counter jumps
+  // whole periods towards maxLoops, so the ray reaches maxLoops in the same
+  // state as the reference (pushes are credited per skipped period).
+  uint64_t snap_path = 0;
+  uint32_t snap_iter = 0, snap_scale = 0, snap_tmin = 0, snap_tmax = 0, snap_pushes = 0;
 
   for (;;) {
     const float se = exp2i(scale - smax);  // scale_exp2
     const float h = se * 0.5f;             // childSizeHalf
     ++iter;
+    if (iter >= 128u) {
+      if ((iter & (iter - 1u)) == 0u) {
+        snap_iter = iter; snap_scale = (uint32_t)scale; snap_path = path;
+        snap_tmin = __float_as_uint(tmin); snap_tmax = __float_as_uint(tmax); snap_pushes = r.pushes;
+      } else if (snap_iter != 0u && (uint32_t)scale == snap_scale && path == snap_path &&
+                 __float_as_uint(tmin) == snap_tmin && __float_as_uint(tmax) == snap_tmax &&
+                 iter < S.max_loops) {
+        // the state repeats with period p: jump whole periods towards maxLoops
+        const uint32_t p = iter - snap_iter;
+        const uint32_t skip = (S.max_loops - iter) / p;
+        r.pushes += skip * (r.pushes - snap_pushes);
+        iter += skip * p;
+        snap_iter = 0u;
+      }
+    }
     if (iter == S.max_loops) {
       if (POLICY == 0) {  // maxLoops returns the current parent as a hit
         r.node = nidx;
@@ -256,8 +284,8 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(TracePools S, const Trace
   }
   const TraceFrame& F = frames[lo];
   const uint32_t lt = tile - F.tile_begin;
-  const uint32_t x = (lt % F.tiles_x) * 8u + (lane & 7u);
-  const uint32_t y = (lt / F.tiles_x) * 8u + (lane >> 3);
+  const uint32_t x = (lt % F.tiles_x) * kTileW + (lane % kTileW);
+  const uint32_t y = (lt / F.tiles_x) * kTileH + (lane / kTileW);
   if (x >= F.cam.width || y >= F.cam.height) return;
 
   const float o[3] = {F.cam.position[0], F.cam.position[1], F.cam.position[2]};

@@ -135,3 +135,38 @@ def test_axis_aligned_and_inside_cameras(menger_mm):
         for policy in ("treelet", "cpu3"):
             _say(f"special camera {i} policy={policy} zero-components={int(np.count_nonzero(d == 0))}")
             _compare(svo.render(mm, cam, policy), ora.render(cam, policy), f"special camera {i} {policy}")
+
+
+@pytest.fixture(scope="module")
+def terrain11(scene_dir):
+    sc = workloads.terrain_scene(depth=11)
+    base = str(scene_dir / "terrain11")
+    sc.write(base)
+    return sc, base
+
+
+@pytest.mark.parametrize("policy", ["treelet", "cpu3"])
+def test_full_size_terrain_1080p_matches_oracle(terrain11, policy):
+    """The bench frame itself (depth-11 terrain, 4096-node treelets, 1920x1080):
+    bit-identical to the oracle, including the rays that cycle until maxLoops
+    (the kernel skips whole periods of such cycles)."""
+    import oracle
+    sc, base = terrain11
+    ib = workloads.incore_bytes_for(sc)
+    mm = svo.TreeletMemoryManagerHip(sc, ib, device=0, resident_all=True)
+    ora = oracle.OracleScene(base, ib)
+    cam = workloads.terrain_camera(workloads.TERRAIN_CAMERA_SEED, 1920, 1080)
+    _say(f"terrain d11 1080p policy={policy}")
+    gpu = svo.render(mm, cam, policy)
+    ref = ora.render(cam, policy)
+    _compare(gpu, ref, f"terrain d11 1080p {policy}")
+    max_loops = 14 * 14 * 5 if policy == "treelet" else 12 * 12 * 5
+    cycling = int(np.count_nonzero(ref["loops"] >= max_loops))
+    _say(f"rays reaching maxLoops: {cycling}; boundary rays: {int(ref['boundary'].sum())}")
+    if policy == "treelet":
+        assert cycling > 0  # the frame exercises the cycle-skip path
+        other = ora.render(cam, "cpu3")
+        diff = (gpu["node"] != other["node"]) | (gpu["treelet"] != other["treelet"])
+        flagged = (ref["boundary"] | other["boundary"]).astype(bool)
+        _say(f"treelet policy vs CPU-caster policy: {int(diff.sum())} differing pixels, all flagged")
+        assert not np.any(diff & ~flagged)
