@@ -295,6 +295,71 @@ typedef struct svo_frame_out {
 svo_status svo_render(svo_mm* mm, const svo_camera* cams, uint32_t n_cams,
                       const svo_frame_out* outs, int policy, void* stream);
 
+/* ------------------------------------------ multi-view launches, graphs */
+
+/* One view of a launch: the scene (its residency handle), the camera and
+ * the outputs.  A launch may mix views of different scenes and sizes;
+ * all handles must live on the renderer's device. */
+typedef struct svo_view {
+  svo_mm* mm;
+  svo_camera camera;
+  svo_frame_out out;
+} svo_view;
+
+/* Launch shape chosen by svo_autotune (or set by hand):
+ *   tile_width      : pixels per wavefront tile row, 8, 16 or 32 (the tile
+ *                     is tile_width x 64/tile_width)
+ *   waves_per_group : wavefronts per workgroup, 1, 2 or 4
+ *   stack_layout    : SVO_STACK_LDS (shared memory, [level][lane]) or
+ *                     SVO_STACK_PRIVATE (per-lane private array) */
+enum { SVO_STACK_LDS = 0, SVO_STACK_PRIVATE = 1 };
+typedef struct svo_launch_config {
+  uint32_t tile_width;
+  uint32_t waves_per_group;
+  uint32_t stack_layout;
+  uint32_t reserved;
+} svo_launch_config;
+
+/* A renderer owns the device-side view descriptors of its launches (a ring,
+ * so that launches on different streams can be in flight together) and the
+ * launch shape.  Default shape: 16-wide tiles, 4 waves, LDS stack. */
+typedef struct svo_renderer svo_renderer;
+svo_status svo_renderer_create(int device, svo_renderer** out);
+void svo_renderer_destroy(svo_renderer* r);
+svo_status svo_renderer_set_config(svo_renderer* r, const svo_launch_config* c);
+svo_status svo_renderer_get_config(const svo_renderer* r, svo_launch_config* c);
+
+/* One launch rendering every view (the renderToBuffer launch, generalised
+ * to many cameras and scenes).  Asynchronous on `stream`. */
+svo_status svo_render_views(svo_renderer* r, const svo_view* views, uint32_t n,
+                            int policy, void* stream);
+
+/* Mean ms per launch over `iters` back-to-back launches (HIP events on
+ * `stream`). */
+svo_status svo_render_views_timed(svo_renderer* r, const svo_view* views,
+                                  uint32_t n, int policy, void* stream,
+                                  uint32_t iters, float* mean_ms);
+
+/* Times every supported launch shape on these views (`iters` launches each,
+ * after one warm-up) and installs the fastest in the renderer.  Results are
+ * identical for every shape; only the speed differs. */
+svo_status svo_autotune(svo_renderer* r, const svo_view* views, uint32_t n,
+                        int policy, void* stream, uint32_t iters,
+                        svo_launch_config* best, float* best_ms);
+
+/* A batch is a multi-view launch captured once into a hipGraph and replayed
+ * with svo_batch_launch.  Its view descriptors live in device memory owned by
+ * the batch; svo_batch_set_cameras rewrites the camera uniforms (same frame
+ * sizes) with an asynchronous copy on `stream`, ordered before the next
+ * launch on that stream. */
+typedef struct svo_batch svo_batch;
+svo_status svo_batch_create(svo_renderer* r, const svo_view* views, uint32_t n,
+                            int policy, void* stream, svo_batch** out);
+svo_status svo_batch_set_cameras(svo_batch* b, const svo_camera* cams,
+                                 uint32_t n, void* stream);
+svo_status svo_batch_launch(svo_batch* b, void* stream);
+void svo_batch_destroy(svo_batch* b);
+
 /* ------------------------------------------- device memory and streams */
 
 svo_status svo_device_count(int* count);

@@ -22,11 +22,12 @@ from typing import Dict, List, Optional, Sequence
 import numpy as np
 
 from . import _capi
-from ._capi import (BACKGROUND_RGBA, MISS, SVO_BUILD_PACK_BFS, SVO_POLICY_CPU3, SVO_POLICY_TREELET, SvoError,
-                    check, lib, svo_camera, svo_frame_out)
+from ._capi import (BACKGROUND_RGBA, MISS, SVO_BUILD_PACK_BFS, SVO_POLICY_CPU3, SVO_POLICY_TREELET, SVO_STACK_LDS,
+                    SVO_STACK_PRIVATE, SvoError, check, lib, svo_camera, svo_frame_out, svo_launch_config, svo_view)
 
 __all__ = [
     "Scene", "TreeletMemoryManagerHip", "Camera", "DeviceBuffer", "Stream", "Frame", "render_to_buffer",
+    "View", "Renderer", "Batch", "LaunchConfig", "LAUNCH_SHAPES",
     "device_count", "look_at", "SvoError", "POLICIES", "MISS", "BACKGROUND_RGBA",
 ]
 
@@ -351,3 +352,108 @@ def ray_dirs(cam: svo_camera) -> np.ndarray:
     check(lib.svo_debug_ray_dirs(ctypes.byref(cam), buf.ptr, None), "svo_debug_ray_dirs")
     check(lib.svo_device_synchronize(), "svo_device_synchronize")
     return buf.to_numpy(np.float32).reshape(cam.height, cam.width, 3)
+
+
+# ------------------------------------------------------------ multi-view launches
+
+LaunchConfig = svo_launch_config
+STACKS = {"lds": SVO_STACK_LDS, "private": SVO_STACK_PRIVATE}
+#: every (tile_width, waves_per_group, stack_layout) the kernels are built for
+LAUNCH_SHAPES = [(t, w, s) for t in (8, 16, 32) for w in (1, 2, 4) for s in ("lds", "private")]
+
+
+@dataclass
+class View:
+    """One camera of one resident scene rendered into one frame."""
+    mm: TreeletMemoryManagerHip
+    camera: svo_camera
+    frame: Frame
+
+    def c_view(self) -> svo_view:
+        return svo_view(self.mm.handle, self.camera, self.frame.out())
+
+
+def _view_array(views: Sequence[View]):
+    if not views:
+        raise ValueError("at least one view required")
+    return (svo_view * len(views))(*[v.c_view() for v in views])
+
+
+class Renderer:
+    """Launch state of multi-view rendering on one device: the launch shape
+    and a ring of device-side view descriptors."""
+
+    def __init__(self, device: int = 0):
+        self._h = ctypes.c_void_p()
+        check(lib.svo_renderer_create(device, ctypes.byref(self._h)), "svo_renderer_create")
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib.svo_renderer_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    @property
+    def handle(self) -> ctypes.c_void_p:
+        return self._h
+
+    @property
+    def config(self):
+        c = svo_launch_config()
+        check(lib.svo_renderer_get_config(self._h, ctypes.byref(c)), "svo_renderer_get_config")
+        return (c.tile_width, c.waves_per_group, "lds" if c.stack_layout == SVO_STACK_LDS else "private")
+
+    @config.setter
+    def config(self, shape) -> None:
+        if shape is None:
+            check(lib.svo_renderer_set_config(self._h, None), "svo_renderer_set_config")
+            return
+        t, w, s = shape
+        c = svo_launch_config(t, w, STACKS[s] if isinstance(s, str) else s, 0)
+        check(lib.svo_renderer_set_config(self._h, ctypes.byref(c)), "svo_renderer_set_config")
+
+    def render(self, views: Sequence[View], policy="treelet", stream: Optional[Stream] = None) -> None:
+        """Enqueues ONE launch rendering every view."""
+        check(lib.svo_render_views(self._h, _view_array(views), len(views), _policy(policy),
+                                   stream.handle if stream else None), "svo_render_views")
+
+    def timed(self, views: Sequence[View], policy="treelet", stream: Optional[Stream] = None, iters: int = 10) -> float:
+        ms = ctypes.c_float()
+        check(lib.svo_render_views_timed(self._h, _view_array(views), len(views), _policy(policy),
+                                         stream.handle if stream else None, iters, ctypes.byref(ms)),
+              "svo_render_views_timed")
+        return ms.value
+
+    def autotune(self, views: Sequence[View], policy="treelet", stream: Optional[Stream] = None, iters: int = 3):
+        """Times every launch shape on these views, keeps the fastest and
+        returns ((tile_width, waves, stack), ms)."""
+        best, ms = svo_launch_config(), ctypes.c_float()
+        check(lib.svo_autotune(self._h, _view_array(views), len(views), _policy(policy),
+                               stream.handle if stream else None, iters, ctypes.byref(best), ctypes.byref(ms)),
+              "svo_autotune")
+        return self.config, ms.value
+
+
+class Batch:
+    """A multi-view launch captured once into a hipGraph; replay with
+    :meth:`launch`, move the cameras with :meth:`set_cameras`."""
+
+    def __init__(self, renderer: Renderer, views: Sequence[View], policy="treelet", stream: Optional[Stream] = None):
+        self._renderer = renderer          # keeps the renderer alive
+        self._views = list(views)          # keeps the frames and residency handles alive
+        self._h = ctypes.c_void_p()
+        check(lib.svo_batch_create(renderer.handle, _view_array(views), len(views), _policy(policy),
+                                   stream.handle if stream else None, ctypes.byref(self._h)), "svo_batch_create")
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib.svo_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def set_cameras(self, cams: Sequence[svo_camera], stream: Optional[Stream] = None) -> None:
+        check(lib.svo_batch_set_cameras(self._h, _cam_array(cams), len(cams), stream.handle if stream else None),
+              "svo_batch_set_cameras")
+        for v, c in zip(self._views, cams):
+            v.camera = c
+
+    def launch(self, stream: Optional[Stream] = None) -> None:
+        check(lib.svo_batch_launch(self._h, stream.handle if stream else None), "svo_batch_launch")

@@ -21,6 +21,7 @@ SVO_ERR_ARG, SVO_ERR_IO, SVO_ERR_FORMAT, SVO_ERR_HIP, SVO_ERR_NOMEM, SVO_ERR_STA
 SVO_POLICY_TREELET = 0
 SVO_POLICY_CPU3 = 2
 SVO_BUILD_PACK_BFS = 1
+SVO_STACK_LDS, SVO_STACK_PRIVATE = 0, 1
 MISS = 0xFFFFFFFF
 BACKGROUND_RGBA = 0x334D33FF
 
@@ -59,6 +60,14 @@ class svo_frame_out(ctypes.Structure):
     _fields_ = [("t", vp), ("node", vp), ("treelet", vp), ("rgba", vp), ("pushes", vp)]
 
 
+class svo_view(ctypes.Structure):
+    _fields_ = [("mm", vp), ("camera", svo_camera), ("out", svo_frame_out)]
+
+
+class svo_launch_config(ctypes.Structure):
+    _fields_ = [("tile_width", u32), ("waves_per_group", u32), ("stack_layout", u32), ("reserved", u32)]
+
+
 # name -> (restype, argtypes); every function declared in include/svo_hip.h
 SIGNATURES = {
     "svo_last_error": (ctypes.c_char_p, []),
@@ -86,6 +95,17 @@ SIGNATURES = {
     "svo_mm_read_host_incore": (ctypes.c_int, [vp, u64, u64, vp, vp]),
     "svo_mm_read_device_incore": (ctypes.c_int, [vp, u64, u64, vp, vp]),
     "svo_render": (ctypes.c_int, [vp, P(svo_camera), u32, P(svo_frame_out), ctypes.c_int, vp]),
+    "svo_renderer_create": (ctypes.c_int, [ctypes.c_int, P(vp)]),
+    "svo_renderer_destroy": (None, [vp]),
+    "svo_renderer_set_config": (ctypes.c_int, [vp, P(svo_launch_config)]),
+    "svo_renderer_get_config": (ctypes.c_int, [vp, P(svo_launch_config)]),
+    "svo_render_views": (ctypes.c_int, [vp, P(svo_view), u32, ctypes.c_int, vp]),
+    "svo_render_views_timed": (ctypes.c_int, [vp, P(svo_view), u32, ctypes.c_int, vp, u32, P(f32)]),
+    "svo_autotune": (ctypes.c_int, [vp, P(svo_view), u32, ctypes.c_int, vp, u32, P(svo_launch_config), P(f32)]),
+    "svo_batch_create": (ctypes.c_int, [vp, P(svo_view), u32, ctypes.c_int, vp, P(vp)]),
+    "svo_batch_set_cameras": (ctypes.c_int, [vp, P(svo_camera), u32, vp]),
+    "svo_batch_launch": (ctypes.c_int, [vp, vp]),
+    "svo_batch_destroy": (None, [vp]),
     "svo_device_count": (ctypes.c_int, [P(ctypes.c_int)]),
     "svo_set_device": (ctypes.c_int, [ctypes.c_int]),
     "svo_device_malloc": (ctypes.c_int, [P(vp), ctypes.c_size_t]),

@@ -10,6 +10,8 @@
 // its six faces touches an empty voxel or the cube boundary.  A ray that
 // starts outside the solid meets the same first voxel in the shell as in the
 // solid, and the shell keeps the node count near the surface area.
+#include <omp.h>
+
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -179,13 +181,18 @@ extern "C" svo_status svo_scene_build_spheres(uint32_t depth, uint32_t count, fl
     set_error("spheres: bad parameters (depth %u, count %u, r [%g,%g])", depth, count, r_min, r_max);
     return SVO_ERR_ARG;
   }
-  const uint32_t n = 1u << depth;
+  const int64_t n = 1ll << depth;
   struct Sphere { double x, y, z, r; };
   std::vector<Sphere> sp(count);
   for (uint32_t s = 0; s < count; ++s) {
     auto u = [&](uint32_t f) { return (double)(mix64(mix64(seed) ^ (4ull * s + f)) >> 11) * (1.0 / 9007199254740992.0); };
     sp[s] = Sphere{u(0), u(1), u(2), r_min + (r_max - r_min) * u(3)};  // centre in [0,1)^3
   }
+  // voxel (i,j,k) is filled by sphere S iff its centre lies strictly inside S
+  auto in_sphere = [&](const Sphere& S, int64_t i, int64_t j, int64_t k) {
+    const double x = (i + 0.5) / n - S.x, y = (j + 0.5) / n - S.y, z = (k + 0.5) / n - S.z;
+    return x * x + y * y + z * z < S.r * S.r;
+  };
   // uniform grid of sphere indices, cell edge >= 2 r_max
   const int gc = std::max(1, (int)std::floor(1.0 / (2.0 * r_max)));
   std::vector<std::vector<uint32_t>> grid((size_t)gc * gc * gc);
@@ -193,34 +200,77 @@ extern "C" svo_status svo_scene_build_spheres(uint32_t depth, uint32_t count, fl
   for (uint32_t s = 0; s < count; ++s) grid[((size_t)cell(sp[s].x) * gc + cell(sp[s].y)) * gc + cell(sp[s].z)].push_back(s);
   auto filled = [&](int64_t i, int64_t j, int64_t k) {
     if (i < 0 || j < 0 || k < 0 || i >= n || j >= n || k >= n) return false;
-    const double x = (i + 0.5) / n, y = (j + 0.5) / n, z = (k + 0.5) / n;
-    const int cx = cell(x), cy = cell(y), cz = cell(z);
+    const int cx = cell((i + 0.5) / n), cy = cell((j + 0.5) / n), cz = cell((k + 0.5) / n);
     for (int a = std::max(0, cx - 1); a <= std::min(gc - 1, cx + 1); ++a)
       for (int b = std::max(0, cy - 1); b <= std::min(gc - 1, cy + 1); ++b)
         for (int c = std::max(0, cz - 1); c <= std::min(gc - 1, cz + 1); ++c)
-          for (uint32_t s : grid[((size_t)a * gc + b) * gc + c]) {
-            const double dx = x - sp[s].x, dy = y - sp[s].y, dz = z - sp[s].z;
-            if (dx * dx + dy * dy + dz * dz < sp[s].r * sp[s].r) return true;
-          }
+          for (uint32_t s : grid[((size_t)a * gc + b) * gc + c])
+            if (in_sphere(sp[s], i, j, k)) return true;
     return false;
   };
-  std::vector<uint64_t> codes;
-  for (uint32_t s = 0; s < count; ++s) {
-    const Sphere& S = sp[s];
-    const int64_t i0 = std::max<int64_t>(0, (int64_t)std::floor((S.x - S.r) * n) - 1), i1 = std::min<int64_t>(n - 1, (int64_t)std::ceil((S.x + S.r) * n) + 1);
-    const int64_t j0 = std::max<int64_t>(0, (int64_t)std::floor((S.y - S.r) * n) - 1), j1 = std::min<int64_t>(n - 1, (int64_t)std::ceil((S.y + S.r) * n) + 1);
-    const int64_t k0 = std::max<int64_t>(0, (int64_t)std::floor((S.z - S.r) * n) - 1), k1 = std::min<int64_t>(n - 1, (int64_t)std::ceil((S.z + S.r) * n) + 1);
-    for (int64_t i = i0; i <= i1; ++i)
-      for (int64_t j = j0; j <= j1; ++j)
+  // j-range of the voxels of column (i,k) inside S, exact w.r.t. in_sphere;
+  // returns false if the column misses S
+  auto column = [&](const Sphere& S, int64_t i, int64_t k, int64_t* lo, int64_t* hi) {
+    if (i < 0 || k < 0 || i >= n || k >= n) return false;
+    const double dx = (i + 0.5) / n - S.x, dz = (k + 0.5) / n - S.z;
+    const double rem = S.r * S.r - dx * dx - dz * dz;
+    if (!(rem > 0)) return false;
+    const double sq = std::sqrt(rem);
+    int64_t a = (int64_t)std::floor((S.y - sq) * n - 0.5), b = (int64_t)std::ceil((S.y + sq) * n - 0.5);
+    a = std::max<int64_t>(0, a - 1);
+    b = std::min<int64_t>(n - 1, b + 1);
+    while (a <= b && !in_sphere(S, i, a, k)) ++a;
+    while (b >= a && !in_sphere(S, i, b, k)) --b;
+    if (a > b) return false;
+    *lo = a;
+    *hi = b;
+    return true;
+  };
+  const int nthreads = std::max(1, omp_get_max_threads());
+  std::vector<std::vector<uint64_t>> found(nthreads);
+#pragma omp parallel num_threads(nthreads)
+  {
+    std::vector<uint64_t>& local = found[omp_get_thread_num()];
+#pragma omp for schedule(dynamic, 1)
+    for (int64_t s = 0; s < (int64_t)count; ++s) {
+      const Sphere& S = sp[s];
+      const int64_t i0 = std::max<int64_t>(0, (int64_t)std::floor((S.x - S.r) * n) - 1);
+      const int64_t i1 = std::min<int64_t>(n - 1, (int64_t)std::ceil((S.x + S.r) * n) + 1);
+      const int64_t k0 = std::max<int64_t>(0, (int64_t)std::floor((S.z - S.r) * n) - 1);
+      const int64_t k1 = std::min<int64_t>(n - 1, (int64_t)std::ceil((S.z + S.r) * n) + 1);
+      for (int64_t i = i0; i <= i1; ++i)
         for (int64_t k = k0; k <= k1; ++k) {
-          const double x = (i + 0.5) / n - S.x, y = (j + 0.5) / n - S.y, z = (k + 0.5) / n - S.z;
-          if (!(x * x + y * y + z * z < S.r * S.r)) continue;
-          if (filled(i - 1, j, k) && filled(i + 1, j, k) && filled(i, j - 1, k) && filled(i, j + 1, k) &&
-              filled(i, j, k - 1) && filled(i, j, k + 1))
-            continue;
-          codes.push_back(morton_encode((uint32_t)i, (uint32_t)j, (uint32_t)k, depth));
+          int64_t lo, hi;
+          if (!column(S, i, k, &lo, &hi)) continue;
+          // voxels of this column whose horizontal neighbours all lie in S
+          // are candidates only if a vertical neighbour leaves S (the ends)
+          int64_t in_lo = lo + 1, in_hi = hi - 1;
+          const int64_t nb[4][2] = {{i - 1, k}, {i + 1, k}, {i, k - 1}, {i, k + 1}};
+          for (const auto& q : nb) {
+            int64_t a, b;
+            if (!column(S, q[0], q[1], &a, &b)) { in_lo = 1; in_hi = 0; break; }
+            in_lo = std::max(in_lo, a);
+            in_hi = std::min(in_hi, b);
+          }
+          for (int64_t j = lo; j <= hi; ++j) {
+            if (j >= in_lo && j <= in_hi) { j = in_hi; continue; }
+            const int64_t nbr[6][3] = {{i - 1, j, k}, {i + 1, j, k}, {i, j - 1, k}, {i, j + 1, k}, {i, j, k - 1}, {i, j, k + 1}};
+            bool shell = false;
+            for (const auto& v : nbr) {
+              const bool inside_cube = v[0] >= 0 && v[1] >= 0 && v[2] >= 0 && v[0] < n && v[1] < n && v[2] < n;
+              if (inside_cube && in_sphere(S, v[0], v[1], v[2])) continue;
+              if (!filled(v[0], v[1], v[2])) { shell = true; break; }
+            }
+            if (shell) local.push_back(morton_encode((uint32_t)i, (uint32_t)j, (uint32_t)k, (uint32_t)depth));
+          }
         }
+    }
   }
+  std::vector<uint64_t> codes;
+  size_t total = 0;
+  for (const auto& v : found) total += v.size();
+  codes.reserve(total);
+  for (auto& v : found) { codes.insert(codes.end(), v.begin(), v.end()); std::vector<uint64_t>().swap(v); }
   radix_sort_u64(codes, 3 * depth);
   codes.erase(std::unique(codes.begin(), codes.end()), codes.end());
   std::vector<uint32_t> attr(2 * codes.size());

@@ -66,11 +66,7 @@ struct svo_mm {
   uint32_t* d_colors = nullptr;
   uint32_t* d_normals = nullptr;
   int32_t* d_slot_gid = nullptr;
-  TraceFrame* d_frames = nullptr;
-  uint32_t frames_cap = 0;
-  TraceFrame* h_frames = nullptr;  // pinned staging for frame descriptors
-  hipEvent_t frames_ready = nullptr;
-  bool frames_pending = false;
+  svo_renderer* renderer = nullptr;  // default renderer of svo_render
 
   bool host_only() const { return device < 0; }
 
@@ -79,14 +75,12 @@ struct svo_mm {
     int cur = 0;
     hipGetDevice(&cur);
     hipSetDevice(device);
-    if (frames_pending) hipEventSynchronize(frames_ready);
-    if (frames_ready) hipEventDestroy(frames_ready);
+    if (renderer) svo_renderer_destroy(renderer);
+    hipDeviceSynchronize();
     hipFree(d_nodes);
     hipFree(d_colors);
     hipFree(d_normals);
     hipFree(d_slot_gid);
-    hipFree(d_frames);
-    if (h_frames) hipHostFree(h_frames);
     hipSetDevice(cur);
   }
 
@@ -311,7 +305,6 @@ extern "C" svo_status svo_mm_create(const svo_scene* scene, uint64_t incore_byte
     set_error("svo_mm_create: hipMalloc of %llu B of pools failed", (unsigned long long)(pool_nodes * 16 + slots * 4));
     return SVO_ERR_NOMEM;
   }
-  HIP_TRY(hipEventCreateWithFlags(&mm->frames_ready, hipEventDisableTiming));
   // initClBuffer: the whole incore buffer goes up once
   r = mm->upload_range(0, mm->num_slots, nullptr, nullptr);
   if (r != SVO_OK) return r;
@@ -417,12 +410,25 @@ extern "C" svo_status svo_mm_read_device_incore(const svo_mm* mm, uint64_t first
   return SVO_OK;
 }
 
-// ------------------------------------------------------------ rendering
+// ------------------------------------------------------------ render hooks
 
-namespace {
+namespace svo_impl {
 
-svo_status make_pools(const svo_mm* mm, int policy, TracePools* P) {
+int mm_device(const svo_mm* mm) { return mm->device; }
+
+svo_renderer* mm_default_renderer(svo_mm* mm) {
+  if (!mm->renderer && svo_renderer_create(mm->device, &mm->renderer) != SVO_OK) return nullptr;
+  return mm->renderer;
+}
+
+svo_status mm_trace_pools(const svo_mm* mm, int policy, TracePools* P) {
+  if (mm->host_only()) { set_error("residency handle has no device pools"); return SVO_ERR_STATE; }
+  if (!mm->dirty.empty()) {
+    set_error("%zu incore slots not uploaded; call svo_mm_update_device first", mm->dirty.size());
+    return SVO_ERR_STATE;
+  }
   const uint32_t D = mm->scene->max_depth;
+  *P = TracePools{};
   P->nodes = mm->d_nodes;
   P->colors = mm->d_colors;
   P->slot_gid = mm->d_slot_gid;
@@ -454,43 +460,14 @@ svo_status make_pools(const svo_mm* mm, int policy, TracePools* P) {
   return SVO_OK;
 }
 
-svo_status stage_frames(svo_mm* mm, const svo_camera* cams, uint32_t n, const svo_frame_out* outs, hipStream_t st,
-                        uint32_t* num_tiles) {
-  if (mm->frames_pending) {
-    HIP_TRY(hipEventSynchronize(mm->frames_ready));
-    mm->frames_pending = false;
-  }
-  if (n > mm->frames_cap) {
-    if (mm->d_frames) HIP_TRY(hipFree(mm->d_frames));
-    if (mm->h_frames) HIP_TRY(hipHostFree(mm->h_frames));
-    mm->d_frames = nullptr;
-    mm->h_frames = nullptr;
-    uint32_t cap = std::max(n, 64u);
-    HIP_TRY(hipMalloc(&mm->d_frames, cap * sizeof(TraceFrame)));
-    HIP_TRY(hipHostMalloc(&mm->h_frames, cap * sizeof(TraceFrame), hipHostMallocDefault));
-    mm->frames_cap = cap;
-  }
-  uint64_t tiles = 0;
-  for (uint32_t i = 0; i < n; ++i) {
-    const svo_camera& c = cams[i];
-    if (c.width == 0 || c.height == 0 || c.width > 16384 || c.height > 16384) {
-      set_error("camera %u: frame %ux%u out of range", i, c.width, c.height);
-      return SVO_ERR_ARG;
-    }
-    TraceFrame& F = mm->h_frames[i];
-    F.cam = c;
-    F.out = outs[i];
-    F.tile_begin = (uint32_t)tiles;
-    F.tiles_x = (c.width + kTileW - 1) / kTileW;
-    F.pad[0] = F.pad[1] = 0;
-    tiles += (uint64_t)F.tiles_x * ((c.height + kTileH - 1) / kTileH);
-  }
-  if (tiles > 0x7FFFFFFFull) { set_error("launch of %llu tiles too large", (unsigned long long)tiles); return SVO_ERR_ARG; }
-  HIP_TRY(hipMemcpyAsync(mm->d_frames, mm->h_frames, n * sizeof(TraceFrame), hipMemcpyHostToDevice, st));
-  HIP_TRY(hipEventRecord(mm->frames_ready, st));
-  mm->frames_pending = true;
-  *num_tiles = (uint32_t)tiles;
-  return SVO_OK;
+}  // namespace svo_impl
+
+namespace {
+
+std::vector<svo_view> make_views(svo_mm* mm, const svo_camera* cams, uint32_t n, const svo_frame_out* outs) {
+  std::vector<svo_view> v(n);
+  for (uint32_t i = 0; i < n; ++i) v[i] = svo_view{mm, cams[i], outs[i]};
+  return v;
 }
 
 }  // namespace
@@ -499,50 +476,20 @@ extern "C" svo_status svo_render(svo_mm* mm, const svo_camera* cams, uint32_t n_
                                  int policy, void* stream) {
   if (!mm || !cams || !outs || n_cams == 0) { set_error("svo_render: null argument or no camera"); return SVO_ERR_ARG; }
   if (mm->host_only()) { set_error("svo_render: handle has no device pools"); return SVO_ERR_STATE; }
-  if (!mm->dirty.empty()) { set_error("svo_render: %zu slots not uploaded; call svo_mm_update_device first", mm->dirty.size()); return SVO_ERR_STATE; }
-  TracePools P;
-  svo_status r = make_pools(mm, policy, &P);
-  if (r != SVO_OK) return r;
-  HIP_TRY(hipSetDevice(mm->device));
-  hipStream_t st = (hipStream_t)stream;
-  uint32_t tiles = 0;
-  r = stage_frames(mm, cams, n_cams, outs, st, &tiles);
-  if (r != SVO_OK) return r;
-  int e = launch_trace(P, mm->d_frames, n_cams, tiles, policy, stream);
-  if (e != 0) { set_error("trace launch failed: %s", hipGetErrorString((hipError_t)e)); return SVO_ERR_HIP; }
-  return SVO_OK;
+  svo_renderer* r = mm_default_renderer(mm);
+  if (!r) return SVO_ERR_HIP;
+  const std::vector<svo_view> v = make_views(mm, cams, n_cams, outs);
+  return svo_render_views(r, v.data(), n_cams, policy, stream);
 }
 
 extern "C" svo_status svo_render_timed(svo_mm* mm, const svo_camera* cams, uint32_t n_cams, const svo_frame_out* outs,
                                        int policy, void* stream, uint32_t iters, float* mean_ms) {
-  if (!mean_ms || iters == 0) { set_error("svo_render_timed: bad arguments"); return SVO_ERR_ARG; }
   if (!mm || !cams || !outs || n_cams == 0) { set_error("svo_render_timed: null argument or no camera"); return SVO_ERR_ARG; }
   if (mm->host_only()) { set_error("svo_render_timed: handle has no device pools"); return SVO_ERR_STATE; }
-  if (!mm->dirty.empty()) { set_error("svo_render_timed: slots not uploaded"); return SVO_ERR_STATE; }
-  TracePools P;
-  svo_status r = make_pools(mm, policy, &P);
-  if (r != SVO_OK) return r;
-  HIP_TRY(hipSetDevice(mm->device));
-  hipStream_t st = (hipStream_t)stream;
-  uint32_t tiles = 0;
-  r = stage_frames(mm, cams, n_cams, outs, st, &tiles);
-  if (r != SVO_OK) return r;
-  hipEvent_t a, b;
-  HIP_TRY(hipEventCreate(&a));
-  HIP_TRY(hipEventCreate(&b));
-  HIP_TRY(hipEventRecord(a, st));
-  for (uint32_t i = 0; i < iters; ++i) {
-    int e = launch_trace(P, mm->d_frames, n_cams, tiles, policy, stream);
-    if (e != 0) { set_error("trace launch failed: %s", hipGetErrorString((hipError_t)e)); return SVO_ERR_HIP; }
-  }
-  HIP_TRY(hipEventRecord(b, st));
-  HIP_TRY(hipEventSynchronize(b));
-  float ms = 0;
-  HIP_TRY(hipEventElapsedTime(&ms, a, b));
-  hipEventDestroy(a);
-  hipEventDestroy(b);
-  *mean_ms = ms / iters;
-  return SVO_OK;
+  svo_renderer* r = mm_default_renderer(mm);
+  if (!r) return SVO_ERR_HIP;
+  const std::vector<svo_view> v = make_views(mm, cams, n_cams, outs);
+  return svo_render_views_timed(r, v.data(), n_cams, policy, stream, iters, mean_ms);
 }
 
 // ------------------------------------------------------------ device utilities

@@ -1,0 +1,333 @@
+// Launch side of the renderer: view descriptors staged through pinned memory
+// into a small device ring, multi-view launches, event timing, the launch
+// shape autotuner and hipGraph-captured batches.
+//
+// Replaces the per-frame renderToBuffer enqueue of the OpenCL app
+// (projects/raycast_serialized_opencl/src/raycast_serialized_opencl.cpp:
+// 729-749), generalised to many cameras and scenes per launch.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <memory>
+#include <vector>
+
+#include "svo_internal.h"
+#include "trace.h"
+
+using namespace svo_impl;
+
+#define HIP_TRY(expr)                                                   \
+  do {                                                                  \
+    hipError_t e_ = (expr);                                             \
+    if (e_ != hipSuccess) {                                             \
+      set_error("%s failed: %s", #expr, hipGetErrorString(e_));         \
+      return SVO_ERR_HIP;                                               \
+    }                                                                   \
+  } while (0)
+
+namespace {
+
+constexpr uint32_t kRing = 8;         // launches in flight per renderer
+constexpr uint32_t kMaxViews = 1u << 16;
+
+// Host image of the descriptors of one launch.
+struct Staged {
+  std::vector<TraceFrame> frames;
+  uint32_t num_tiles = 0;
+  uint32_t stack_levels = 1;
+};
+
+svo_status stage(const svo_view* views, uint32_t n, int policy, const LaunchShape& shape, int device, Staged* out) {
+  if (!views || n == 0 || n > kMaxViews) { set_error("need 1..%u views", kMaxViews); return SVO_ERR_ARG; }
+  const uint32_t tw = shape.tile_w, th = 64u / shape.tile_w;
+  out->frames.resize(n);
+  uint64_t tiles = 0;
+  uint32_t levels = 1;
+  for (uint32_t i = 0; i < n; ++i) {
+    const svo_view& v = views[i];
+    if (!v.mm) { set_error("view %u has no residency handle", i); return SVO_ERR_ARG; }
+    if (mm_device(v.mm) != device) { set_error("view %u lives on device %d, renderer on %d", i, mm_device(v.mm), device); return SVO_ERR_ARG; }
+    const svo_camera& c = v.camera;
+    if (c.width == 0 || c.height == 0 || c.width > 16384 || c.height > 16384) {
+      set_error("view %u: frame %ux%u out of range", i, c.width, c.height);
+      return SVO_ERR_ARG;
+    }
+    TraceFrame& F = out->frames[i];
+    svo_status r = mm_trace_pools(v.mm, policy, &F.pools);
+    if (r != SVO_OK) return r;
+    F.cam = c;
+    F.out = v.out;
+    F.tile_begin = (uint32_t)tiles;
+    F.tiles_x = (c.width + tw - 1) / tw;
+    F.pad[0] = F.pad[1] = 0;
+    tiles += (uint64_t)F.tiles_x * ((c.height + th - 1) / th);
+    levels = std::max(levels, F.pools.stack_levels);
+  }
+  if (tiles > 0x7FFFFFFFull) { set_error("launch of %llu tiles too large", (unsigned long long)tiles); return SVO_ERR_ARG; }
+  out->num_tiles = (uint32_t)tiles;
+  out->stack_levels = levels;
+  return SVO_OK;
+}
+
+struct Slot {
+  TraceFrame* d = nullptr;
+  TraceFrame* h = nullptr;  // pinned
+  uint32_t cap = 0;
+  hipEvent_t done = nullptr;  // recorded after the launch that read d
+  bool pending = false;
+};
+
+svo_status shape_from_config(const svo_launch_config* c, LaunchShape* s) {
+  LaunchShape t{c->tile_width, c->waves_per_group, c->stack_layout};
+  if (!shape_supported(t)) {
+    set_error("launch config (tile %u, waves %u, stack %u) not supported", c->tile_width, c->waves_per_group,
+              c->stack_layout);
+    return SVO_ERR_ARG;
+  }
+  *s = t;
+  return SVO_OK;
+}
+
+}  // namespace
+
+struct svo_renderer {
+  int device = 0;
+  LaunchShape shape{16, 4, SVO_STACK_LDS};
+  Slot ring[kRing];
+  uint32_t next = 0;
+
+  ~svo_renderer() {
+    int cur = 0;
+    hipGetDevice(&cur);
+    hipSetDevice(device);
+    for (Slot& s : ring) {
+      if (s.pending) hipEventSynchronize(s.done);
+      if (s.done) hipEventDestroy(s.done);
+      hipFree(s.d);
+      if (s.h) hipHostFree(s.h);
+    }
+    hipSetDevice(cur);
+  }
+
+  // Copies the staged descriptors into the next ring slot on `st` and
+  // returns its device pointer; the slot is reused only after the launch that
+  // read it has finished.
+  svo_status upload(const Staged& sg, hipStream_t st, Slot** used) {
+    Slot& s = ring[next];
+    next = (next + 1) % kRing;
+    if (s.pending) {
+      HIP_TRY(hipEventSynchronize(s.done));
+      s.pending = false;
+    }
+    const uint32_t n = (uint32_t)sg.frames.size();
+    if (n > s.cap) {
+      if (s.d) HIP_TRY(hipFree(s.d));
+      if (s.h) HIP_TRY(hipHostFree(s.h));
+      s.d = nullptr;
+      s.h = nullptr;
+      s.cap = 0;
+      const uint32_t cap = std::max(n, 64u);
+      HIP_TRY(hipMalloc(&s.d, cap * sizeof(TraceFrame)));
+      HIP_TRY(hipHostMalloc(&s.h, cap * sizeof(TraceFrame), hipHostMallocDefault));
+      s.cap = cap;
+    }
+    if (!s.done) HIP_TRY(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
+    std::copy(sg.frames.begin(), sg.frames.end(), s.h);
+    HIP_TRY(hipMemcpyAsync(s.d, s.h, n * sizeof(TraceFrame), hipMemcpyHostToDevice, st));
+    *used = &s;
+    return SVO_OK;
+  }
+
+  svo_status launch(const LaunchShape& sh, const Staged& sg, Slot* s, int policy, hipStream_t st) {
+    int e = launch_trace(sh, s->d, (uint32_t)sg.frames.size(), sg.num_tiles, sg.stack_levels, policy, st);
+    if (e != 0) { set_error("trace launch failed: %s", hipGetErrorString((hipError_t)e)); return SVO_ERR_HIP; }
+    HIP_TRY(hipEventRecord(s->done, st));
+    s->pending = true;
+    return SVO_OK;
+  }
+
+  svo_status timed(const LaunchShape& sh, const svo_view* views, uint32_t n, int policy, hipStream_t st,
+                   uint32_t warmup, uint32_t iters, float* ms) {
+    Staged sg;
+    svo_status r = stage(views, n, policy, sh, device, &sg);
+    if (r != SVO_OK) return r;
+    Slot* s = nullptr;
+    r = upload(sg, st, &s);
+    if (r != SVO_OK) return r;
+    for (uint32_t i = 0; i < warmup; ++i)
+      if ((r = launch(sh, sg, s, policy, st)) != SVO_OK) return r;
+    hipEvent_t a, b;
+    HIP_TRY(hipEventCreate(&a));
+    HIP_TRY(hipEventCreate(&b));
+    HIP_TRY(hipEventRecord(a, st));
+    for (uint32_t i = 0; i < iters; ++i)
+      if ((r = launch(sh, sg, s, policy, st)) != SVO_OK) return r;
+    HIP_TRY(hipEventRecord(b, st));
+    HIP_TRY(hipEventSynchronize(b));
+    float total = 0;
+    HIP_TRY(hipEventElapsedTime(&total, a, b));
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    *ms = total / (float)iters;
+    return SVO_OK;
+  }
+};
+
+extern "C" svo_status svo_renderer_create(int device, svo_renderer** out) {
+  if (!out) { set_error("svo_renderer_create: out is null"); return SVO_ERR_ARG; }
+  *out = nullptr;
+  int ndev = 0;
+  HIP_TRY(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) { set_error("svo_renderer_create: device %d not present (%d devices)", device, ndev); return SVO_ERR_ARG; }
+  svo_renderer* r = new svo_renderer();
+  r->device = device;
+  *out = r;
+  return SVO_OK;
+}
+
+extern "C" void svo_renderer_destroy(svo_renderer* r) { delete r; }
+
+extern "C" svo_status svo_renderer_set_config(svo_renderer* r, const svo_launch_config* c) {
+  if (!r) { set_error("svo_renderer_set_config: null renderer"); return SVO_ERR_ARG; }
+  if (!c) { r->shape = LaunchShape{16, 4, SVO_STACK_LDS}; return SVO_OK; }
+  return shape_from_config(c, &r->shape);
+}
+
+extern "C" svo_status svo_renderer_get_config(const svo_renderer* r, svo_launch_config* c) {
+  if (!r || !c) { set_error("svo_renderer_get_config: null argument"); return SVO_ERR_ARG; }
+  *c = svo_launch_config{r->shape.tile_w, r->shape.waves, r->shape.stack, 0};
+  return SVO_OK;
+}
+
+extern "C" svo_status svo_render_views(svo_renderer* r, const svo_view* views, uint32_t n, int policy, void* stream) {
+  if (!r) { set_error("svo_render_views: null renderer"); return SVO_ERR_ARG; }
+  HIP_TRY(hipSetDevice(r->device));
+  Staged sg;
+  svo_status st = stage(views, n, policy, r->shape, r->device, &sg);
+  if (st != SVO_OK) return st;
+  Slot* s = nullptr;
+  if ((st = r->upload(sg, (hipStream_t)stream, &s)) != SVO_OK) return st;
+  return r->launch(r->shape, sg, s, policy, (hipStream_t)stream);
+}
+
+extern "C" svo_status svo_render_views_timed(svo_renderer* r, const svo_view* views, uint32_t n, int policy,
+                                             void* stream, uint32_t iters, float* mean_ms) {
+  if (!r || !mean_ms || iters == 0) { set_error("svo_render_views_timed: bad arguments"); return SVO_ERR_ARG; }
+  HIP_TRY(hipSetDevice(r->device));
+  return r->timed(r->shape, views, n, policy, (hipStream_t)stream, 0, iters, mean_ms);
+}
+
+extern "C" svo_status svo_autotune(svo_renderer* r, const svo_view* views, uint32_t n, int policy, void* stream,
+                                   uint32_t iters, svo_launch_config* best, float* best_ms) {
+  if (!r || iters == 0) { set_error("svo_autotune: bad arguments"); return SVO_ERR_ARG; }
+  HIP_TRY(hipSetDevice(r->device));
+  const uint32_t tiles[] = {8, 16, 32}, waves[] = {1, 2, 4}, stacks[] = {SVO_STACK_LDS, SVO_STACK_PRIVATE};
+  float bm = 0;
+  LaunchShape bs{};
+  bool found = false;
+  for (uint32_t t : tiles)
+    for (uint32_t w : waves)
+      for (uint32_t k : stacks) {
+        const LaunchShape sh{t, w, k};
+        float ms = 0;
+        svo_status st = r->timed(sh, views, n, policy, (hipStream_t)stream, 1, iters, &ms);
+        if (st != SVO_OK) return st;
+        if (!found || ms < bm) { bm = ms; bs = sh; found = true; }
+      }
+  r->shape = bs;
+  if (best) *best = svo_launch_config{bs.tile_w, bs.waves, bs.stack, 0};
+  if (best_ms) *best_ms = bm;
+  return SVO_OK;
+}
+
+// ------------------------------------------------------------ batches
+
+struct svo_batch {
+  svo_renderer* r = nullptr;
+  int policy = 0;
+  LaunchShape shape{};
+  Staged sg;
+  TraceFrame* d = nullptr;
+  TraceFrame* h = nullptr;  // pinned
+  hipEvent_t copied = nullptr;
+  bool pending = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+
+  ~svo_batch() {
+    int cur = 0;
+    hipGetDevice(&cur);
+    hipSetDevice(r->device);
+    if (pending) hipEventSynchronize(copied);
+    hipDeviceSynchronize();
+    if (exec) hipGraphExecDestroy(exec);
+    if (graph) hipGraphDestroy(graph);
+    if (copied) hipEventDestroy(copied);
+    hipFree(d);
+    if (h) hipHostFree(h);
+    hipSetDevice(cur);
+  }
+};
+
+extern "C" svo_status svo_batch_create(svo_renderer* r, const svo_view* views, uint32_t n, int policy, void* stream,
+                                       svo_batch** out) {
+  if (!r || !out) { set_error("svo_batch_create: null argument"); return SVO_ERR_ARG; }
+  *out = nullptr;
+  HIP_TRY(hipSetDevice(r->device));
+  std::unique_ptr<svo_batch> b(new svo_batch());
+  b->r = r;
+  b->policy = policy;
+  b->shape = r->shape;
+  svo_status st = stage(views, n, policy, b->shape, r->device, &b->sg);
+  if (st != SVO_OK) return st;
+  const size_t bytes = n * sizeof(TraceFrame);
+  HIP_TRY(hipMalloc(&b->d, bytes));
+  HIP_TRY(hipHostMalloc(&b->h, bytes, hipHostMallocDefault));
+  HIP_TRY(hipEventCreateWithFlags(&b->copied, hipEventDisableTiming));
+  std::copy(b->sg.frames.begin(), b->sg.frames.end(), b->h);
+  HIP_TRY(hipMemcpyAsync(b->d, b->h, bytes, hipMemcpyHostToDevice, (hipStream_t)stream));
+  HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+  // capture the launch on a private stream (the default stream cannot capture)
+  hipStream_t cap;
+  HIP_TRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+  hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    int le = launch_trace(b->shape, b->d, n, b->sg.num_tiles, b->sg.stack_levels, policy, cap);
+    hipError_t e2 = hipStreamEndCapture(cap, &b->graph);
+    e = le != 0 ? (hipError_t)le : e2;
+  }
+  hipStreamDestroy(cap);
+  if (e != hipSuccess) { set_error("graph capture of the batch launch failed: %s", hipGetErrorString(e)); return SVO_ERR_HIP; }
+  HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  *out = b.release();
+  return SVO_OK;
+}
+
+extern "C" svo_status svo_batch_set_cameras(svo_batch* b, const svo_camera* cams, uint32_t n, void* stream) {
+  if (!b || !cams) { set_error("svo_batch_set_cameras: null argument"); return SVO_ERR_ARG; }
+  if (n != b->sg.frames.size()) { set_error("batch has %zu views, got %u cameras", b->sg.frames.size(), n); return SVO_ERR_ARG; }
+  for (uint32_t i = 0; i < n; ++i)
+    if (cams[i].width != b->sg.frames[i].cam.width || cams[i].height != b->sg.frames[i].cam.height) {
+      set_error("camera %u changes the frame size of the batch", i);
+      return SVO_ERR_ARG;
+    }
+  HIP_TRY(hipSetDevice(b->r->device));
+  if (b->pending) {
+    HIP_TRY(hipEventSynchronize(b->copied));
+    b->pending = false;
+  }
+  for (uint32_t i = 0; i < n; ++i) b->h[i].cam = cams[i];
+  HIP_TRY(hipMemcpyAsync(b->d, b->h, n * sizeof(TraceFrame), hipMemcpyHostToDevice, (hipStream_t)stream));
+  HIP_TRY(hipEventRecord(b->copied, (hipStream_t)stream));
+  b->pending = true;
+  return SVO_OK;
+}
+
+extern "C" svo_status svo_batch_launch(svo_batch* b, void* stream) {
+  if (!b) { set_error("svo_batch_launch: null batch"); return SVO_ERR_ARG; }
+  HIP_TRY(hipSetDevice(b->r->device));
+  HIP_TRY(hipGraphLaunch(b->exec, (hipStream_t)stream));
+  return SVO_OK;
+}
+
+extern "C" void svo_batch_destroy(svo_batch* b) { delete b; }

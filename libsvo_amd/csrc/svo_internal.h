@@ -79,6 +79,15 @@ struct svo_scene {
 
 namespace svo_impl {
 
+struct TracePools;
+
+// Residency handles as seen by the renderer (memory_manager.cpp).
+int mm_device(const svo_mm* mm);
+svo_renderer* mm_default_renderer(svo_mm* mm);
+// Pools and traversal parameters of `mm` for `policy`; fails while slots
+// wait for upload or when the handle has no device pools.
+svo_status mm_trace_pools(const svo_mm* mm, int policy, TracePools* out);
+
 // Builds the scene from sorted, unique leaf codes at `depth` and two
 // attribute words per leaf.  Consumes `codes` (cleared on return).
 svo_status build_scene(uint32_t depth, std::vector<uint64_t>& codes,

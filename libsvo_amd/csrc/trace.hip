@@ -1,10 +1,12 @@
 // Serialized-SVO treelet traversal for gfx950.
 //
-// One ray per lane, one 16x4 pixel tile per wavefront, four wavefronts per
-// workgroup.  The per-ray stack lives in LDS ([level][lane], conflict-free),
-// 4 bytes per level: the node index of each parent on the path.  The child
-// index taken from each parent goes to a 64-bit path register (3 bits per
-// level); the parent node itself is re-read on a pop.
+// One ray per lane, one pixel tile per wavefront (16x4 by default), a few
+// wavefronts per workgroup; every view of a launch may use its own scene.
+// The per-ray stack holds 4 bytes per level, the node index of each parent on
+// the path, in LDS ([level][lane], conflict-free) or in private memory (the
+// launch shape picks; the autotuner measures both).  The child index taken
+// from each parent goes to a 64-bit path register (3 bits per level); the
+// parent node itself is re-read on a pop.
 // A parent's centre, its tmax and its tmin after the pop are recomputed
 // bit-exactly instead of stored:
 //   * centres are dyadic, so parent = child - (-0.5 + bit) * 2^(s - scaleMax)
@@ -28,10 +30,7 @@
 namespace svo_impl {
 namespace {
 
-constexpr int kWaves = 4;
-constexpr int kBlock = 64 * kWaves;
 constexpr int kMaxScale = 16;  // stack levels; scaleMax <= 16
-static_assert(kTileW * kTileH == 64, "one wavefront per tile");
 
 // OpenCL max/min as specified for floats: max -> y if x < y else x,
 // min -> y if y < x else x.
@@ -98,12 +97,29 @@ __device__ __forceinline__ float node_tmax(const float c[3], float h, const floa
   return cl_min(t1[0], cl_min(t1[1], t1[2]));
 }
 
+// Per-ray stack of parent indices.
+template <int STACK, int BLOCK>
+struct RayStack;
+template <int BLOCK>
+struct RayStack<SVO_STACK_LDS, BLOCK> {  // LDS, [level][lane]
+  uint32_t* base;
+  uint32_t tid;
+  __device__ __forceinline__ void put(int l, uint32_t v) { base[l * BLOCK + tid] = v; }
+  __device__ __forceinline__ uint32_t get(int l) const { return base[l * BLOCK + tid]; }
+};
+template <int BLOCK>
+struct RayStack<SVO_STACK_PRIVATE, BLOCK> {  // private (scratch-backed) array
+  uint32_t v[kMaxScale];
+  __device__ __forceinline__ void put(int l, uint32_t x) { v[l] = x; }
+  __device__ __forceinline__ uint32_t get(int l) const { return v[l]; }
+};
+
 // POLICY 0: treeletRenderer_rgba.cl::sample (:201-382).
 // POLICY 2: CpuRaycasterSingleRay3::traversSvo
 //           (src/incore/CpuRaycasterSingleRay3.cpp:158-387).
 //
-// Stack: s_idx[lvl][lane] holds the node index of the parent left by a push,
-// lvl = scale - min_push_scale; the child index taken at each push is kept in
+// Stack: level lvl = scale - min_push_scale holds the node index of the
+// parent left by a push; the child index taken at each push is kept in
 // `path`, 3 bits per level.  A pop re-reads the parent node (a cache hit: it
 // was fetched a few iterations earlier).  Every node value therefore comes
 // from the one conditional load at the end of an iteration, and its first
@@ -111,9 +127,8 @@ __device__ __forceinline__ float node_tmax(const float c[3], float h, const floa
 // arithmetic that overlaps the fetch.  (Running a lane's pops back to back
 // before its next child step was measured slower: it serialises the pop
 // chains of a wave.)
-template <int POLICY>
-__device__ Hit traverse(const TracePools& S, const float o[3], float d[3], float ratio, uint32_t* s_idx,
-                        uint32_t tid) {
+template <int POLICY, typename Stack>
+__device__ Hit traverse(const TracePools& S, const float o[3], float d[3], float ratio, Stack& stk) {
   Hit r{0xFFFFFFFFu, __int_as_float(0x7F800000), 0u};
   float t_min, t_max;
   if (!root_box(o, d, t_min, t_max)) return r;
@@ -233,7 +248,7 @@ __device__ Hit traverse(const TracePools& S, const float o[3], float d[3], float
         }
         // push
         ++r.pushes;
-        s_idx[(scale - mps) * kBlock + tid] = nidx;
+        stk.put(scale - mps, nidx);
         path = (path << 3) | ci;
         --scale;
         c[0] = cc[0]; c[1] = cc[1]; c[2] = cc[2];
@@ -248,7 +263,7 @@ __device__ Hit traverse(const TracePools& S, const float o[3], float d[3], float
       ++scale;
       if (scale >= smax) return r;  // left the root: miss, t = root tMax
       const float t_child = tmax;
-      fidx = s_idx[(scale - mps) * kBlock + tid];
+      fidx = stk.get(scale - mps);
       fetch = true;
       const uint32_t ci = (uint32_t)(path & 7u);
       path >>= 3;
@@ -266,14 +281,16 @@ __device__ Hit traverse(const TracePools& S, const float o[3], float d[3], float
   }
 }
 
-template <int POLICY>
-__global__ __launch_bounds__(kBlock) void trace_kernel(TracePools S, const TraceFrame* __restrict__ frames,
-                                                       uint32_t num_frames, uint32_t num_tiles) {
-  extern __shared__ uint32_t s_idx[];
+template <int POLICY, int TW, int WAVES, int STACK>
+__global__ __launch_bounds__(64 * WAVES) void trace_kernel(const TraceFrame* __restrict__ frames, uint32_t num_frames,
+                                                           uint32_t num_tiles) {
+  constexpr int kBlock = 64 * WAVES;
+  constexpr uint32_t TH = 64u / TW;
+  extern __shared__ uint32_t s_stack[];
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = tid & 63u;
   const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const uint32_t tile = blockIdx.x * kWaves + wave;
+  const uint32_t tile = blockIdx.x * WAVES + wave;
   if (tile >= num_tiles) return;
 
   uint32_t lo = 0, hi = num_frames;
@@ -284,14 +301,20 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(TracePools S, const Trace
   }
   const TraceFrame& F = frames[lo];
   const uint32_t lt = tile - F.tile_begin;
-  const uint32_t x = (lt % F.tiles_x) * kTileW + (lane % kTileW);
-  const uint32_t y = (lt / F.tiles_x) * kTileH + (lane / kTileW);
+  const uint32_t x = (lt % F.tiles_x) * TW + (lane % TW);
+  const uint32_t y = (lt / F.tiles_x) * TH + (lane / TW);
   if (x >= F.cam.width || y >= F.cam.height) return;
+  const TracePools& S = F.pools;
 
   const float o[3] = {F.cam.position[0], F.cam.position[1], F.cam.position[2]};
   float d[3];
   pixel_ray(F.cam, x, y, d);
-  const Hit hit = traverse<POLICY>(S, o, d, F.cam.t_scale_ratio, s_idx, tid);
+  RayStack<STACK, kBlock> stk;
+  if constexpr (STACK == SVO_STACK_LDS) {
+    stk.base = s_stack;
+    stk.tid = tid;
+  }
+  const Hit hit = traverse<POLICY>(S, o, d, F.cam.t_scale_ratio, stk);
 
   const size_t pix = (size_t)y * F.cam.width + x;
   uint32_t tree = 0xFFFFFFFFu, rgba = 0x334D33FFu;
@@ -325,7 +348,38 @@ __global__ __launch_bounds__(256) void copy_kernel(uint4* __restrict__ dst, cons
     dst[i] = src[i];
 }
 
+using TraceFn = void (*)(const TraceFrame*, uint32_t, uint32_t);
+
+template <int POLICY, int TW, int WAVES>
+TraceFn pick_stack(uint32_t stack) {
+  return stack == SVO_STACK_PRIVATE ? trace_kernel<POLICY, TW, WAVES, SVO_STACK_PRIVATE>
+                                    : trace_kernel<POLICY, TW, WAVES, SVO_STACK_LDS>;
+}
+
+template <int POLICY, int TW>
+TraceFn pick_waves(uint32_t waves, uint32_t stack) {
+  switch (waves) {
+    case 1: return pick_stack<POLICY, TW, 1>(stack);
+    case 2: return pick_stack<POLICY, TW, 2>(stack);
+    default: return pick_stack<POLICY, TW, 4>(stack);
+  }
+}
+
+template <int POLICY>
+TraceFn pick_tile(const LaunchShape& s) {
+  switch (s.tile_w) {
+    case 8: return pick_waves<POLICY, 8>(s.waves, s.stack);
+    case 32: return pick_waves<POLICY, 32>(s.waves, s.stack);
+    default: return pick_waves<POLICY, 16>(s.waves, s.stack);
+  }
+}
+
 }  // namespace
+
+bool shape_supported(const LaunchShape& s) {
+  return (s.tile_w == 8 || s.tile_w == 16 || s.tile_w == 32) && (s.waves == 1 || s.waves == 2 || s.waves == 4) &&
+         (s.stack == SVO_STACK_LDS || s.stack == SVO_STACK_PRIVATE);
+}
 
 int launch_copy(void* dst, const void* src, size_t bytes, void* stream) {
   const size_t n = bytes / 16;
@@ -335,17 +389,15 @@ int launch_copy(void* dst, const void* src, size_t bytes, void* stream) {
   return (int)hipGetLastError();
 }
 
-int launch_trace(const TracePools& pools, const TraceFrame* frames_dev, uint32_t num_frames, uint32_t num_tiles,
-                 int policy, void* stream) {
+int launch_trace(const LaunchShape& shape, const TraceFrame* frames_dev, uint32_t num_frames, uint32_t num_tiles,
+                 uint32_t stack_levels, int policy, void* stream) {
   if (num_tiles == 0) return 0;
-  if (pools.stack_levels > kMaxScale) return (int)hipErrorInvalidValue;
-  const dim3 grid((num_tiles + kWaves - 1) / kWaves), block(kBlock);
-  const size_t lds = (size_t)pools.stack_levels * kBlock * sizeof(uint32_t);
-  hipStream_t st = (hipStream_t)stream;
-  if (policy == SVO_POLICY_CPU3)
-    hipLaunchKernelGGL(trace_kernel<2>, grid, block, lds, st, pools, frames_dev, num_frames, num_tiles);
-  else
-    hipLaunchKernelGGL(trace_kernel<0>, grid, block, lds, st, pools, frames_dev, num_frames, num_tiles);
+  if (!shape_supported(shape) || stack_levels > (uint32_t)kMaxScale) return (int)hipErrorInvalidValue;
+  const TraceFn fn = policy == SVO_POLICY_CPU3 ? pick_tile<2>(shape) : pick_tile<0>(shape);
+  const uint32_t block = 64u * shape.waves;
+  const dim3 grid((num_tiles + shape.waves - 1) / shape.waves);
+  const size_t lds = shape.stack == SVO_STACK_LDS ? (size_t)stack_levels * block * sizeof(uint32_t) : 0;
+  hipLaunchKernelGGL(fn, grid, dim3(block), lds, (hipStream_t)stream, frames_dev, num_frames, num_tiles);
   return (int)hipGetLastError();
 }
 
