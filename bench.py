@@ -1,12 +1,21 @@
 #!/usr/bin/env python3
-"""Single-frame throughput of the serialized-SVO treelet ray caster.
+"""Throughput of the serialized-SVO treelet ray caster on one GPU.
 
-Workload: depth-11 value-noise terrain (2048^3 leaf grid) in 4096-node
-treelets, all resident in HBM; one 1920x1080 FP32 primary-ray frame from a
-seeded camera 0.1 above the surface; treelet (OpenCL renderer) traversal
-policy, full depth (no LOD stop).  A step is one launch that renders the
-frame with the fused colour / depth / node / treelet epilogue; inputs are
-resident in HBM before the timed region starts.
+Headline (``value``): depth-11 value-noise terrain (2048^3 leaf grid) in
+4096-node treelets, all resident in HBM; one 1920x1080 FP32 primary-ray frame
+from a seeded camera 0.1 above the surface; treelet (OpenCL renderer)
+traversal policy, full depth (no LOD stop).  A step is one launch that
+renders the frame with the fused colour / depth / node / treelet epilogue;
+inputs are resident in HBM before the timed region starts.
+
+Also reported in the same line:
+* ``batched``: 64 views over the depth-11 terrain and a depth-10 sphere
+  cloud, resolutions uniform over {256^2, 512^2, 720p, 1080p}, one launch
+  captured in a hipGraph, launch shape from the autotuner;
+* ``serving``: Poisson request streams over four resident scenes (Menger
+  depth 9, terrain depth 11, sphere cloud depth 10, terrain depth 12) in a
+  64 GB HBM budget, served by the continuous-batching server and by the plain
+  one-frame-per-launch loop: frames/s, Mrays/s, p50/p99 latency.
 
 Prints one JSON line (rank 0, one GPU).  Progress goes to stderr.
 """
@@ -64,6 +73,111 @@ def cpu_baseline(scene, cam, threads, frames=5):
     }
 
 
+def _scene(cache, key):
+    """Builds (once) and makes resident one of the named benchmark scenes."""
+    import libsvo_amd as svo
+    from libsvo_amd import workloads
+    if key not in cache:
+        t0 = time.perf_counter()
+        sc = {"terrain11": lambda: workloads.terrain_scene(depth=11),
+              "terrain12": lambda: workloads.terrain_scene(depth=12),
+              "spheres10": lambda: workloads.sphere_cloud_scene(depth=10),
+              "menger9": lambda: workloads.menger_scene(depth=9, treelet_bytes=32768)}[key]()
+        mm = svo.TreeletMemoryManagerHip(sc, workloads.incore_bytes_for(sc), device=0, resident_all=True)
+        log(f"{key}: built and resident in {time.perf_counter() - t0:.1f}s, {sc.info['num_treelets']} treelets, "
+            f"{mm.info['device_bytes'] / 2**30:.2f} GiB in HBM")
+        cache[key] = (sc, mm)
+    return cache[key]
+
+
+def batched_section(cache, stream, policy, launches=10):
+    """64 views over two scenes in one launch, replayed from a hipGraph."""
+    import libsvo_amd as svo
+    from libsvo_amd import workloads
+    kinds = ["terrain", "spheres"]
+    mms = [_scene(cache, "terrain11")[1], _scene(cache, "spheres10")[1]]
+    spec = workloads.batched_views(kinds)
+    views = [svo.View(mms[k], cam, svo.Frame.alloc(cam.width, cam.height)) for k, cam in spec]
+    rays = sum(v.camera.width * v.camera.height for v in views)
+    r = svo.Renderer(0)
+    default_ms = r.timed(views, policy, stream, iters=3)
+    t0 = time.perf_counter()
+    shape, tuned_ms = r.autotune(views, policy, stream, iters=2)
+    log(f"batched: {len(views)} views, {rays} rays; default shape {default_ms:.3f} ms, autotuned {shape} "
+        f"{tuned_ms:.3f} ms (tuning took {time.perf_counter() - t0:.1f}s)")
+    b = svo.Batch(r, views, policy, stream)
+    b.launch(stream)
+    stream.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(launches):
+        b.launch(stream)
+    stream.synchronize()
+    graph_ms = (time.perf_counter() - t0) * 1e3 / launches
+    # the same views, one launch per view
+    t0 = time.perf_counter()
+    for v in views:
+        r.render([v], policy, stream)
+    stream.synchronize()
+    single_ms = (time.perf_counter() - t0) * 1e3
+    log(f"batched: graph replay {graph_ms:.3f} ms/launch = {rays / graph_ms / 1e3:.1f} Mrays/s; one launch per "
+        f"view {single_ms:.3f} ms = {rays / single_ms / 1e3:.1f} Mrays/s")
+    return {
+        "workload": "64 views, depth-11 terrain + depth-10 sphere cloud (2000 spheres), resolutions uniform over "
+                    "{256^2, 512^2, 1280x720, 1920x1080}, one hipGraph-captured launch",
+        "views": len(views), "rays_per_launch": rays, "launch_shape": list(shape),
+        "ms_per_launch": round(graph_ms, 4), "mrays_per_s": round(rays / graph_ms / 1e3, 1),
+        "default_shape_ms": round(default_ms, 4), "autotuned_ms": round(tuned_ms, 4),
+        "one_launch_per_view_mrays_per_s": round(rays / single_ms / 1e3, 1),
+        "seed": workloads.BATCH_SEED,
+    }
+
+
+def serving_section(cache, rates, seconds, policy):
+    """Poisson request streams over four resident scenes: continuous batching
+    against the plain one-frame-per-launch loop."""
+    import libsvo_amd as svo
+    from libsvo_amd import workloads
+    keys = ["menger9", "terrain11", "spheres10", "terrain12"]
+    kinds = ["menger", "terrain", "spheres", "terrain"]
+    budget = 64 << 30
+    r = svo.Renderer(0)
+    modes = {"continuous": dict(max_batch_views=64, max_inflight=4, max_batch_rays=16 << 20),
+             "plain": dict(max_batch_views=1, max_inflight=1)}
+    servers = {}
+    for name, kw in modes.items():
+        srv = svo.Server(r, hbm_budget=budget, frame_pool=4 << 30, policy=policy, **kw)
+        for k in keys:
+            srv.add_scene(_scene(cache, k)[1])
+        servers[name] = srv
+    scene_bytes = sum(_scene(cache, k)[1].info["device_bytes"] for k in keys)
+    results = []
+    for rate in rates:
+        reqs = workloads.serving_requests(kinds, rate, max(50, int(rate * seconds)))
+        row = {"rate_req_s": rate, "requests": len(reqs)}
+        for name, srv in servers.items():
+            st, comp = srv.replay(reqs)
+            row[name] = {"frames_per_s": round(st["frames_per_s"], 1), "mrays_per_s": round(st["mrays_per_s"], 1),
+                         "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3),
+                         "max_ms": round(st["max_ms"], 3), "mean_batch_views": round(st["mean_batch_views"], 2),
+                         "max_inflight": st["max_inflight_seen"]}
+            log(f"serving {rate:.0f} req/s {name}: {row[name]}")
+        results.append(row)
+    # saturation: every request queued at t=0
+    reqs = workloads.serving_requests(kinds, 1e9, 2000)
+    sat = {}
+    for name, srv in servers.items():
+        st, _ = srv.replay(reqs)
+        sat[name] = {"frames_per_s": round(st["frames_per_s"], 1), "mrays_per_s": round(st["mrays_per_s"], 1)}
+        log(f"serving saturation {name}: {sat[name]}")
+    return {
+        "workload": "Menger d9 + terrain d11 + sphere cloud d10 + terrain d12 resident, Poisson arrivals, "
+                    "resolutions 256^2 40% / 720p 40% / 1080p 15% / 4K 5%, random cameras; latency = "
+                    "completion on device - arrival",
+        "hbm_budget_bytes": budget, "scene_bytes": scene_bytes, "rates": results, "saturation": sat,
+        "seed": workloads.SERVING_SEED, "seconds_per_rate": seconds,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -75,6 +189,10 @@ def main():
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serving-seconds", type=float, default=3.0, help="length of each request stream")
+    ap.add_argument("--serving-rates", default="200,1000,2000", help="Poisson arrival rates (req/s)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="after the timed launches, run one plain device copy of this many MiB "
                          "(known traffic for scaling FETCH_SIZE / WRITE_SIZE counters)")
@@ -163,6 +281,12 @@ def main():
             "traffic": None,
         },
     }
+    extra_scenes = {"terrain11": (scene, mm)}
+    if not args.no_batched:
+        out["batched"] = batched_section(extra_scenes, stream, args.policy)
+    if not args.no_serving:
+        rates = [float(r) for r in args.serving_rates.split(",") if r]
+        out["serving"] = serving_section(extra_scenes, rates, args.serving_seconds, args.policy)
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
         log(f"cpu baseline on {threads} threads")

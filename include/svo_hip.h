@@ -360,6 +360,88 @@ svo_status svo_batch_set_cameras(svo_batch* b, const svo_camera* cams,
 svo_status svo_batch_launch(svo_batch* b, void* stream);
 void svo_batch_destroy(svo_batch* b);
 
+/* ------------------------------------------------------------ serving */
+
+/* A render server keeps several residency handles (scenes) on one device and
+ * turns a stream of render requests into multi-view launches as they
+ * arrive (continuous batching).  It has no counterpart in the reference,
+ * whose app renders one camera per frame (raycast_serialized_opencl.cpp:
+ * 684-749); each launch it makes is the svo_render_views launch.
+ *
+ *  - Scenes are admitted by HBM budget: svo_server_add_scene fails with
+ *    SVO_ERR_CAPACITY when the scene's device bytes plus the frame pool
+ *    would exceed hbm_budget_bytes.
+ *  - Requests whose outputs are all null get their outputs (t, node,
+ *    treelet, rgba: 16 bytes per pixel) from the server's frame pool of
+ *    frame_pool_bytes; a request waits (in arrival order) until its frame
+ *    fits.  Results stay in HBM; the frame is returned to the pool when the
+ *    request completes.  Requests with any output set render into the
+ *    caller's buffers.
+ *  - Up to max_inflight launches run at once, each on its own stream; a
+ *    launch takes waiting requests in arrival order up to max_batch_views
+ *    views and max_batch_rays rays (at least one request).
+ *    max_inflight = max_batch_views = 1 is the plain one-frame-per-launch
+ *    loop.
+ * A server is driven from one host thread. */
+typedef struct svo_server svo_server;
+
+typedef struct svo_server_config {
+  uint64_t hbm_budget_bytes;   /* scenes + frame pool; 0 = no limit */
+  uint64_t frame_pool_bytes;   /* output pool for requests without outputs */
+  uint64_t max_batch_rays;     /* 0 = no limit */
+  uint32_t max_batch_views;    /* 1..65536 */
+  uint32_t max_inflight;       /* 1..16 */
+  int32_t policy;              /* SVO_POLICY_* */
+  uint32_t reserved;
+} svo_server_config;
+
+typedef struct svo_request {
+  uint32_t scene;              /* id from svo_server_add_scene */
+  uint32_t reserved;
+  double arrival_s;            /* svo_server_replay: offset from the start */
+  svo_camera camera;
+  svo_frame_out out;           /* all null: outputs from the frame pool */
+} svo_request;
+
+/* Times in seconds since the server was created (online use) or since the
+ * start of the replay. */
+typedef struct svo_completion {
+  uint64_t id;                 /* svo_server_submit id / replay index */
+  double arrival_s, launch_s, done_s;
+  uint32_t batch;              /* launch sequence number */
+  uint32_t batch_views;        /* views in that launch */
+} svo_completion;
+
+typedef struct svo_server_stats {
+  uint64_t requests, rays, batches;
+  uint64_t peak_pool_bytes;
+  uint32_t max_batch_views_seen, max_inflight_seen;
+  double wall_s;               /* first arrival to last completion */
+  double frames_per_s, mrays_per_s;
+  double p50_ms, p99_ms, mean_ms, max_ms;   /* done - arrival */
+  double mean_batch_views;
+} svo_server_stats;
+
+svo_status svo_server_create(svo_renderer* r, const svo_server_config* cfg,
+                             svo_server** out);
+void svo_server_destroy(svo_server* s);
+svo_status svo_server_add_scene(svo_server* s, svo_mm* mm, uint32_t* scene_id);
+
+/* Online use: queue a request now (its arrival time), then call
+ * svo_server_poll repeatedly; it retires finished launches, starts new ones
+ * and returns up to `cap` completions. */
+svo_status svo_server_submit(svo_server* s, const svo_request* req, uint64_t* id);
+svo_status svo_server_poll(svo_server* s, svo_completion* done, uint32_t cap,
+                           uint32_t* n_done);
+/* Requests queued or in flight. */
+svo_status svo_server_pending(const svo_server* s, uint32_t* n);
+
+/* Benchmark driver: submits reqs[i] at its arrival_s offset in real time
+ * (arrival_s ascending), serves them until all complete and fills one
+ * completion per request (may be null) and the summary. */
+svo_status svo_server_replay(svo_server* s, const svo_request* reqs, uint32_t n,
+                             svo_completion* done, svo_server_stats* stats);
+
 /* ------------------------------------------- device memory and streams */
 
 svo_status svo_device_count(int* count);

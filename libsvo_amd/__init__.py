@@ -27,7 +27,7 @@ from ._capi import (BACKGROUND_RGBA, MISS, SVO_BUILD_PACK_BFS, SVO_POLICY_CPU3, 
 
 __all__ = [
     "Scene", "TreeletMemoryManagerHip", "Camera", "DeviceBuffer", "Stream", "Frame", "render_to_buffer",
-    "View", "Renderer", "Batch", "LaunchConfig", "LAUNCH_SHAPES",
+    "View", "Renderer", "Batch", "LaunchConfig", "LAUNCH_SHAPES", "Server",
     "device_count", "look_at", "SvoError", "POLICIES", "MISS", "BACKGROUND_RGBA",
 ]
 
@@ -457,3 +457,75 @@ class Batch:
 
     def launch(self, stream: Optional[Stream] = None) -> None:
         check(lib.svo_batch_launch(self._h, stream.handle if stream else None), "svo_batch_launch")
+
+
+# ------------------------------------------------------------ serving
+
+class Server:
+    """Continuous-batching render server over resident scenes (one device).
+
+    Requests without a :class:`Frame` render into the server's frame pool
+    (results stay in HBM); requests with one render into it."""
+
+    def __init__(self, renderer: Renderer, hbm_budget: int = 0, frame_pool: int = 1 << 30, max_batch_rays: int = 0,
+                 max_batch_views: int = 64, max_inflight: int = 4, policy="treelet"):
+        self._renderer = renderer
+        self._scenes: List[TreeletMemoryManagerHip] = []
+        self._frames = {}
+        cfg = _capi.svo_server_config(int(hbm_budget), int(frame_pool), int(max_batch_rays), int(max_batch_views),
+                                      int(max_inflight), _policy(policy), 0)
+        self._h = ctypes.c_void_p()
+        check(lib.svo_server_create(renderer.handle, ctypes.byref(cfg), ctypes.byref(self._h)), "svo_server_create")
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib.svo_server_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def add_scene(self, mm: TreeletMemoryManagerHip) -> int:
+        sid = ctypes.c_uint32()
+        check(lib.svo_server_add_scene(self._h, mm.handle, ctypes.byref(sid)), "svo_server_add_scene")
+        self._scenes.append(mm)
+        return sid.value
+
+    @staticmethod
+    def _request(scene: int, cam: svo_camera, arrival_s: float = 0.0, frame: Optional[Frame] = None):
+        out = frame.out() if frame is not None else svo_frame_out()
+        return _capi.svo_request(scene, 0, float(arrival_s), cam, out)
+
+    def submit(self, scene: int, cam: svo_camera, frame: Optional[Frame] = None) -> int:
+        rid = ctypes.c_uint64()
+        req = self._request(scene, cam, 0.0, frame)
+        check(lib.svo_server_submit(self._h, ctypes.byref(req), ctypes.byref(rid)), "svo_server_submit")
+        if frame is not None:
+            self._frames[rid.value] = frame  # kept alive until completion
+        return rid.value
+
+    def poll(self, cap: int = 256) -> List[Dict[str, float]]:
+        buf = (_capi.svo_completion * cap)()
+        n = ctypes.c_uint32()
+        check(lib.svo_server_poll(self._h, buf, cap, ctypes.byref(n)), "svo_server_poll")
+        out = []
+        for c in buf[:n.value]:
+            self._frames.pop(c.id, None)
+            out.append({k: getattr(c, k) for k, _ in _capi.svo_completion._fields_})
+        return out
+
+    @property
+    def pending(self) -> int:
+        n = ctypes.c_uint32()
+        check(lib.svo_server_pending(self._h, ctypes.byref(n)), "svo_server_pending")
+        return n.value
+
+    def replay(self, requests: Sequence[tuple]):
+        """Serves (scene, camera, arrival_s[, frame]) requests in real time.
+        Returns (stats dict, completions array with fields arrival_s,
+        launch_s, done_s, batch, batch_views, in request order)."""
+        n = len(requests)
+        reqs = (_capi.svo_request * n)(*[self._request(*r) for r in requests])
+        done = (_capi.svo_completion * n)()
+        st = _capi.svo_server_stats()
+        check(lib.svo_server_replay(self._h, reqs, n, done, ctypes.byref(st)), "svo_server_replay")
+        stats = {k: getattr(st, k) for k, _ in _capi.svo_server_stats._fields_}
+        comp = np.ctypeslib.as_array(done).copy() if n else np.zeros(0)
+        return stats, comp

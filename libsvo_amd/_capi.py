@@ -64,6 +64,28 @@ class svo_view(ctypes.Structure):
     _fields_ = [("mm", vp), ("camera", svo_camera), ("out", svo_frame_out)]
 
 
+class svo_server_config(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", u64), ("frame_pool_bytes", u64), ("max_batch_rays", u64),
+                ("max_batch_views", u32), ("max_inflight", u32), ("policy", i32), ("reserved", u32)]
+
+
+class svo_request(ctypes.Structure):
+    _fields_ = [("scene", u32), ("reserved", u32), ("arrival_s", f64), ("camera", svo_camera),
+                ("out", svo_frame_out)]
+
+
+class svo_completion(ctypes.Structure):
+    _fields_ = [("id", u64), ("arrival_s", f64), ("launch_s", f64), ("done_s", f64), ("batch", u32),
+                ("batch_views", u32)]
+
+
+class svo_server_stats(ctypes.Structure):
+    _fields_ = [("requests", u64), ("rays", u64), ("batches", u64), ("peak_pool_bytes", u64),
+                ("max_batch_views_seen", u32), ("max_inflight_seen", u32), ("wall_s", f64),
+                ("frames_per_s", f64), ("mrays_per_s", f64), ("p50_ms", f64), ("p99_ms", f64),
+                ("mean_ms", f64), ("max_ms", f64), ("mean_batch_views", f64)]
+
+
 class svo_launch_config(ctypes.Structure):
     _fields_ = [("tile_width", u32), ("waves_per_group", u32), ("stack_layout", u32), ("reserved", u32)]
 
@@ -106,6 +128,13 @@ SIGNATURES = {
     "svo_batch_set_cameras": (ctypes.c_int, [vp, P(svo_camera), u32, vp]),
     "svo_batch_launch": (ctypes.c_int, [vp, vp]),
     "svo_batch_destroy": (None, [vp]),
+    "svo_server_create": (ctypes.c_int, [vp, P(svo_server_config), P(vp)]),
+    "svo_server_destroy": (None, [vp]),
+    "svo_server_add_scene": (ctypes.c_int, [vp, vp, P(u32)]),
+    "svo_server_submit": (ctypes.c_int, [vp, P(svo_request), P(u64)]),
+    "svo_server_poll": (ctypes.c_int, [vp, P(svo_completion), u32, P(u32)]),
+    "svo_server_pending": (ctypes.c_int, [vp, P(u32)]),
+    "svo_server_replay": (ctypes.c_int, [vp, P(svo_request), u32, P(svo_completion), P(svo_server_stats)]),
     "svo_device_count": (ctypes.c_int, [P(ctypes.c_int)]),
     "svo_set_device": (ctypes.c_int, [ctypes.c_int]),
     "svo_device_malloc": (ctypes.c_int, [P(vp), ctypes.c_size_t]),

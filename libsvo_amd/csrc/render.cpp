@@ -26,18 +26,14 @@ using namespace svo_impl;
   } while (0)
 
 namespace {
-
 constexpr uint32_t kRing = 8;         // launches in flight per renderer
 constexpr uint32_t kMaxViews = 1u << 16;
+}  // namespace
 
-// Host image of the descriptors of one launch.
-struct Staged {
-  std::vector<TraceFrame> frames;
-  uint32_t num_tiles = 0;
-  uint32_t stack_levels = 1;
-};
+namespace svo_impl {
 
-svo_status stage(const svo_view* views, uint32_t n, int policy, const LaunchShape& shape, int device, Staged* out) {
+svo_status stage_views(const svo_view* views, uint32_t n, int policy, const LaunchShape& shape, int device,
+                       StagedLaunch* out) {
   if (!views || n == 0 || n > kMaxViews) { set_error("need 1..%u views", kMaxViews); return SVO_ERR_ARG; }
   const uint32_t tw = shape.tile_w, th = 64u / shape.tile_w;
   out->frames.resize(n);
@@ -68,6 +64,10 @@ svo_status stage(const svo_view* views, uint32_t n, int policy, const LaunchShap
   out->stack_levels = levels;
   return SVO_OK;
 }
+
+}  // namespace svo_impl
+
+namespace {
 
 struct Slot {
   TraceFrame* d = nullptr;
@@ -112,7 +112,7 @@ struct svo_renderer {
   // Copies the staged descriptors into the next ring slot on `st` and
   // returns its device pointer; the slot is reused only after the launch that
   // read it has finished.
-  svo_status upload(const Staged& sg, hipStream_t st, Slot** used) {
+  svo_status upload(const StagedLaunch& sg, hipStream_t st, Slot** used) {
     Slot& s = ring[next];
     next = (next + 1) % kRing;
     if (s.pending) {
@@ -138,7 +138,7 @@ struct svo_renderer {
     return SVO_OK;
   }
 
-  svo_status launch(const LaunchShape& sh, const Staged& sg, Slot* s, int policy, hipStream_t st) {
+  svo_status launch(const LaunchShape& sh, const StagedLaunch& sg, Slot* s, int policy, hipStream_t st) {
     int e = launch_trace(sh, s->d, (uint32_t)sg.frames.size(), sg.num_tiles, sg.stack_levels, policy, st);
     if (e != 0) { set_error("trace launch failed: %s", hipGetErrorString((hipError_t)e)); return SVO_ERR_HIP; }
     HIP_TRY(hipEventRecord(s->done, st));
@@ -148,8 +148,8 @@ struct svo_renderer {
 
   svo_status timed(const LaunchShape& sh, const svo_view* views, uint32_t n, int policy, hipStream_t st,
                    uint32_t warmup, uint32_t iters, float* ms) {
-    Staged sg;
-    svo_status r = stage(views, n, policy, sh, device, &sg);
+    StagedLaunch sg;
+    svo_status r = stage_views(views, n, policy, sh, device, &sg);
     if (r != SVO_OK) return r;
     Slot* s = nullptr;
     r = upload(sg, st, &s);
@@ -172,6 +172,11 @@ struct svo_renderer {
     return SVO_OK;
   }
 };
+
+namespace svo_impl {
+LaunchShape renderer_shape(const svo_renderer* r) { return r->shape; }
+int renderer_device(const svo_renderer* r) { return r->device; }
+}  // namespace svo_impl
 
 extern "C" svo_status svo_renderer_create(int device, svo_renderer** out) {
   if (!out) { set_error("svo_renderer_create: out is null"); return SVO_ERR_ARG; }
@@ -202,8 +207,8 @@ extern "C" svo_status svo_renderer_get_config(const svo_renderer* r, svo_launch_
 extern "C" svo_status svo_render_views(svo_renderer* r, const svo_view* views, uint32_t n, int policy, void* stream) {
   if (!r) { set_error("svo_render_views: null renderer"); return SVO_ERR_ARG; }
   HIP_TRY(hipSetDevice(r->device));
-  Staged sg;
-  svo_status st = stage(views, n, policy, r->shape, r->device, &sg);
+  StagedLaunch sg;
+  svo_status st = stage_views(views, n, policy, r->shape, r->device, &sg);
   if (st != SVO_OK) return st;
   Slot* s = nullptr;
   if ((st = r->upload(sg, (hipStream_t)stream, &s)) != SVO_OK) return st;
@@ -246,7 +251,7 @@ struct svo_batch {
   svo_renderer* r = nullptr;
   int policy = 0;
   LaunchShape shape{};
-  Staged sg;
+  StagedLaunch sg;
   TraceFrame* d = nullptr;
   TraceFrame* h = nullptr;  // pinned
   hipEvent_t copied = nullptr;
@@ -278,7 +283,7 @@ extern "C" svo_status svo_batch_create(svo_renderer* r, const svo_view* views, u
   b->r = r;
   b->policy = policy;
   b->shape = r->shape;
-  svo_status st = stage(views, n, policy, b->shape, r->device, &b->sg);
+  svo_status st = stage_views(views, n, policy, b->shape, r->device, &b->sg);
   if (st != SVO_OK) return st;
   const size_t bytes = n * sizeof(TraceFrame);
   HIP_TRY(hipMalloc(&b->d, bytes));

@@ -3,6 +3,7 @@
 #pragma once
 
 #include <cstdint>
+#include <vector>
 
 #include "../../include/svo_hip.h"
 
@@ -53,6 +54,18 @@ bool shape_supported(const LaunchShape& s);
 // stack depth over the views.  Returns a hipError_t value (0 = success).
 int launch_trace(const LaunchShape& shape, const TraceFrame* frames_dev, uint32_t num_frames, uint32_t num_tiles,
                  uint32_t stack_levels, int policy, void* stream);
+
+// Host image of one launch's view descriptors (render.cpp): views in order,
+// tiles numbered consecutively for the given shape.
+struct StagedLaunch {
+  std::vector<TraceFrame> frames;
+  uint32_t num_tiles = 0;
+  uint32_t stack_levels = 1;
+};
+svo_status stage_views(const svo_view* views, uint32_t n, int policy, const LaunchShape& shape, int device,
+                       StagedLaunch* out);
+LaunchShape renderer_shape(const svo_renderer* r);
+int renderer_device(const svo_renderer* r);
 
 // Writes the normalised FP32 ray direction of every pixel of `cam`
 // (3 floats per pixel, row-major) -- used to pin the ray generation.

@@ -11,6 +11,14 @@ half-pixel offset.
 * Terrain     : depth-11 (2048^3) value-noise terrain in 4096-node treelets,
                 1920x1080 rays from 0.1 above the surface at a seeded (x,z),
                 yaw uniform in [0, 2pi), pitch uniform in [-30, 0] degrees.
+* Sphere cloud: depth-10 (1024^3) grid, 2000 spheres, radius uniform in
+                [0.005, 0.05], centres uniform in the unit cube.
+* Batched     : 64 views per launch over the depth-11 terrain and the sphere
+                cloud, resolutions uniform over {256^2, 512^2, 1280x720,
+                1920x1080}.
+* Serving     : Poisson arrivals over Menger depth 9, terrain depth 11,
+                sphere cloud depth 10 and terrain depth 12; resolutions
+                256^2 40 %, 1280x720 40 %, 1920x1080 15 %, 3840x2160 5 %.
 """
 from __future__ import annotations
 
@@ -25,6 +33,9 @@ MENGER_SEED = 1
 MENGER_CAMERA_SEED = 123
 TERRAIN_SEED = 7
 TERRAIN_CAMERA_SEED = 11
+SPHERES_SEED = 5
+BATCH_SEED = 31
+SERVING_SEED = 47
 FAR = 20.0
 FOV = 60.0
 
@@ -66,6 +77,56 @@ def terrain_camera(seed: int, width: int = 1920, height: int = 1080, lod: float 
     pitch = math.radians(rng.uniform(-30.0, 0.0))
     fwd = np.array([math.cos(pitch) * math.cos(yaw), math.sin(pitch), math.cos(pitch) * math.sin(yaw)])
     return look_at(eye, eye + fwd, (0.0, 1.0, 0.0), FOV, width, height, FAR, lod)
+
+
+def sphere_cloud_scene(depth: int = 10, count: int = 2000, r_min: float = 0.005, r_max: float = 0.05,
+                       treelet_bytes: int = 32768, packing: str = "dense", seed: int = SPHERES_SEED) -> Scene:
+    return Scene.spheres(depth, count, r_min, r_max, seed, treelet_bytes, packing)
+
+
+BATCH_RESOLUTIONS = [(256, 256), (512, 512), (1280, 720), (1920, 1080)]
+SERVING_RESOLUTIONS = [(256, 256), (1280, 720), (1920, 1080), (3840, 2160)]
+SERVING_WEIGHTS = [0.40, 0.40, 0.15, 0.05]
+
+
+def scene_camera(kind: str, seed: int, width: int, height: int, lod: float = 0.0):
+    """Seeded camera for a scene kind: terrain cameras fly above the surface,
+    the others orbit the cube."""
+    if kind == "terrain":
+        return terrain_camera(seed, width, height, lod=lod)
+    return sphere_camera(seed, width, height, lod=lod)
+
+
+def batched_views(kinds, n: int = 64, seed: int = BATCH_SEED):
+    """n (scene index, camera) pairs: scene uniform over `kinds`, resolution
+    uniform over BATCH_RESOLUTIONS."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        k = int(rng.integers(len(kinds)))
+        w, h = BATCH_RESOLUTIONS[int(rng.integers(len(BATCH_RESOLUTIONS)))]
+        out.append((k, scene_camera(kinds[k], int(rng.integers(1 << 30)), w, h)))
+    return out
+
+
+def serving_requests(kinds, rate: float, count: int, seed: int = SERVING_SEED, resolutions=None, weights=None):
+    """`count` requests of a Poisson process of `rate` per second: (scene
+    index, camera, arrival_s), scene uniform over `kinds`, resolution drawn
+    from SERVING_RESOLUTIONS with SERVING_WEIGHTS."""
+    rng = np.random.default_rng(seed)
+    res = resolutions or SERVING_RESOLUTIONS
+    wts = np.asarray(weights or SERVING_WEIGHTS, np.float64)
+    gaps = rng.exponential(1.0 / rate, size=count)
+    arrivals = np.cumsum(gaps) - gaps[0]
+    scenes = rng.integers(len(kinds), size=count)
+    picks = rng.choice(len(res), size=count, p=wts / wts.sum())
+    cam_seeds = rng.integers(1 << 30, size=count)
+    out = []
+    for i in range(count):
+        k = int(scenes[i])
+        w, h = res[int(picks[i])]
+        out.append((k, scene_camera(kinds[k], int(cam_seeds[i]), w, h), float(arrivals[i])))
+    return out
 
 
 def incore_bytes_for(scene: Scene, spare_slots: int = 0) -> int:
