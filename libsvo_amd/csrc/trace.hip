@@ -21,6 +21,15 @@
 // Every floating-point expression follows the reference's operation order
 // and is compiled with -ffp-contract=off, so the kernel reproduces the CPU
 // restatement bit for bit.
+//
+// The loop is bound by VALU issue (one wave64 VALU instruction per SIMD
+// every few cycles, several waves per SIMD), so its cost is its dynamic
+// instruction count.  Rays whose reciprocal direction is finite on every
+// axis (all but exactly axis-aligned ones) run traverse_fast: plane
+// parameters from the sign of the direction instead of compare-and-swap, two
+// planes per axis per child step, the parent's far faces only on a pop, and a
+// single loop exit (exits inside the loop make the compiler copy the loop
+// state at every exit).  A wave with any other ray runs the generic loop.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -281,6 +290,214 @@ __device__ Hit traverse(const TracePools& S, const float o[3], float d[3], float
   }
 }
 
+
+// Wave-uniform traversal parameters, held in scalar registers.
+struct Uniform {
+  const __attribute__((address_space(1))) uint64_t* nodes;
+  int smax;
+  int mps;
+  uint32_t max_loops;
+  int lod_scale;  // policy A: scale whose inner nodes stop the descent (-1000 = none)
+  uint32_t pool_nodes;
+};
+
+__device__ __forceinline__ uint32_t sgpr(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
+
+__device__ __forceinline__ Uniform make_uniform(const TracePools& S) {
+  Uniform U;
+  const uint64_t p = (uint64_t)S.nodes;
+  const uint64_t q = ((uint64_t)sgpr((uint32_t)(p >> 32)) << 32) | sgpr((uint32_t)p);
+  U.nodes = (const __attribute__((address_space(1))) uint64_t*)q;
+  U.smax = (int)sgpr(S.scale_max);
+  U.mps = (int)sgpr(S.min_push_scale);
+  U.max_loops = sgpr(S.max_loops);
+  const uint32_t cap = sgpr(S.lod_cap);
+  U.lod_scale = cap != 0u ? U.smax - (int)cap : -1000;
+  U.pool_nodes = sgpr(S.pool_nodes);
+  return U;
+}
+
+// Traversal for rays whose reciprocal direction is finite on every axis
+// (the common case; a direction component that is exactly zero stays zero
+// after the reference's clamp, since sign(0) = 0, and takes the generic
+// loop).  Same states and FP32 operations as the generic loop, with less
+// work per step:
+//   * every plane coordinate (node centre plus or minus a power of two) is
+//     exact, so t(p) = rc*p + mo is the value the reference computes for
+//     that plane however the coordinate was formed;
+//   * with rc finite, t(p) is monotone in p (rounding is monotone), so the
+//     compare-and-swap of the reference (swapToIncreasing) puts the plane on
+//     the near side of the ray into t0 and the far one into t1; where the two
+//     rounded values are equal they are the same non-zero number (a zero t
+//     needs the plane through the origin, the other plane is then non-zero);
+//   * a child box on one axis is bounded by the node's midplane and one face
+//     of the node, so a step evaluates two planes per axis; a pop evaluates
+//     only the parent's far faces (its tcMax).
+template <int POLICY, typename Stack>
+__device__ Hit traverse_fast(const Uniform& U, const float o[3], const float d[3], const float rc[3], const float mo[3],
+                             float t_min, float t_max, float ratio, Stack& stk, Hit r) {
+  const float eps = 0.00001f;
+  bool neg[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) neg[a] = rc[a] < 0.0f;
+
+  int scale = U.smax - 1;
+  float c[3] = {0.0f, 0.0f, 0.0f};
+  float tmin = t_min, tmax = t_max;
+  uint32_t nidx = 0;
+  uint64_t nd = U.nodes[0];
+  uint64_t path = 0;
+  uint32_t iter = 0;
+  uint32_t pushes = 0;
+  uint64_t snap_path = 0;
+  uint32_t snap_iter = 0, snap_scale = 0, snap_tmin = 0, snap_tmax = 0, snap_pushes = 0;
+  // outcome: 0 running, 1 hit at `hit_node` with t = tmin, 2 miss (t = root tMax),
+  // 3 maxLoops, 4 corrupt offset
+  uint32_t outcome = 0;
+  uint32_t hit_node = 0;
+
+  do {
+    const float se = exp2i(scale - U.smax);  // half-edge of the current node
+    ++iter;
+    if (iter >= 128u) {
+      if ((iter & (iter - 1u)) == 0u) {
+        snap_iter = iter; snap_scale = (uint32_t)scale; snap_path = path;
+        snap_tmin = __float_as_uint(tmin); snap_tmax = __float_as_uint(tmax); snap_pushes = pushes;
+      } else if (snap_iter != 0u && (uint32_t)scale == snap_scale && path == snap_path &&
+                 __float_as_uint(tmin) == snap_tmin && __float_as_uint(tmax) == snap_tmax &&
+                 iter < U.max_loops) {
+        const uint32_t p = iter - snap_iter;
+        const uint32_t skip = (U.max_loops - iter) / p;
+        pushes += skip * (pushes - snap_pushes);
+        iter += skip * p;
+        snap_iter = 0u;
+      }
+    }
+    if (iter == U.max_loops) {
+      outcome = 3;
+      break;
+    }
+    bool pop;
+    if (POLICY == 0) pop = (tmax < 0.0f) || !(fabsf(tmin - tmax) > eps);
+    else pop = (tmax < 0.0f) || (fabsf(tmin - tmax) < eps) || !(tmin < tmax);
+
+    bool fetch = false;
+    uint32_t fidx = 0;
+    if (!pop) {
+      const float te = tmin + eps;
+      bool up[3];
+      up[0] = ((o[0] + te * d[0]) - c[0]) > 0.0f;
+      up[1] = ((o[1] + te * d[1]) - c[1]) > 0.0f;
+      up[2] = ((o[2] + te * d[2]) - c[2]) > 0.0f;
+      const uint32_t ci = (up[0] ? 4u : 0u) | (up[1] ? 2u : 0u) | (up[2] ? 1u : 0u);
+      float t0[3], t1[3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const float tmid = rc[a] * c[a] + mo[a];
+        const float tface = rc[a] * (c[a] + (up[a] ? se : -se)) + mo[a];
+        const bool nearside = up[a] == neg[a];  // child on the near side of the midplane
+        t0[a] = nearside ? tface : tmid;
+        t1[a] = nearside ? tmid : tface;
+      }
+      const float tc_min = cl_max(t0[0], cl_max(t0[1], t0[2]));
+      const float tc_max = cl_min(t1[0], cl_min(t1[1], t1[2]));
+      const uint32_t masks = (uint32_t)(nd >> 32);
+      if ((masks >> (8u + ci)) & 1u) {
+        const uint32_t child = nidx + (uint32_t)nd + (uint32_t)__popc((masks >> 8) & ((1u << ci) - 1u));
+        bool stop;
+        if (POLICY == 0) stop = (ratio * tc_min > se) || scale == U.lod_scale;
+        else stop = (ratio * tc_max > se) || scale == 0;
+        if (((masks >> ci) & 1u) || stop) {  // leaf, or inner node at the LOD / depth limit
+          outcome = 1;
+          hit_node = child;
+        } else if (child >= U.pool_nodes) {
+          outcome = 4;
+        } else {
+          ++pushes;
+          stk.put(scale - U.mps, nidx);
+          path = (path << 3) | ci;
+          --scale;
+          const float h = se * 0.5f;
+#pragma unroll
+          for (int a = 0; a < 3; ++a) c[a] = c[a] + (up[a] ? h : -h);
+          tmin = tc_min;
+          tmax = tc_max;
+          fidx = child;
+          fetch = true;
+        }
+      } else {
+        tmin = tc_max;  // advance
+      }
+    } else if (scale + 1 >= U.smax) {
+      outcome = 2;  // left the root
+    } else {
+      ++scale;
+      fidx = stk.get(scale - U.mps);
+      fetch = true;
+      const uint32_t ci = (uint32_t)(path & 7u);
+      path >>= 3;
+      const float sp = se * 2.0f;  // half-edge of the parent
+      float tfar[3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        c[a] = c[a] + (((ci >> (2 - a)) & 1u) ? -se : se);
+        tfar[a] = rc[a] * (c[a] + (neg[a] ? -sp : sp)) + mo[a];
+      }
+      tmin = tmax;
+      tmax = (scale == U.smax - 1) ? t_max : cl_min(tfar[0], cl_min(tfar[1], tfar[2]));
+    }
+    if (fetch) {
+      nidx = fidx;
+      nd = U.nodes[fidx];
+    }
+  } while (outcome == 0);
+
+  r.pushes = pushes;
+  if (outcome == 1) {
+    r.node = hit_node;
+    r.t = tmin;
+  } else if (outcome == 3) {
+    if (POLICY == 0) {  // maxLoops returns the current parent as a hit
+      r.node = nidx;
+      r.t = tmin;
+    }
+  }
+  return r;
+}
+
+// Root-box test, direction clamp and reciprocals, then the fast loop when
+// every active lane of the wave has a finite reciprocal direction, else the
+// generic loop.
+template <int POLICY, typename Stack>
+__device__ Hit trace_ray(const TracePools& S, const float o[3], float d[3], float ratio, Stack& stk) {
+  Hit r{0xFFFFFFFFu, __int_as_float(0x7F800000), 0u};
+  float t_min, t_max;
+  if (!root_box(o, d, t_min, t_max)) return r;
+  float dc[3] = {d[0], d[1], d[2]};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    if (POLICY == 0) {
+      if (fabsf(dc[a]) < 0.0001f) dc[a] = 0.0001f * cl_sign(dc[a]);
+    } else {
+      if (fabsf(dc[a]) < 0.00001f) dc[a] = 0.00001f * cl_sign(dc[a]) * 10.0f;
+    }
+  }
+  float rc[3], mo[3];
+  bool finite = true;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    rc[a] = 1.0f / dc[a];
+    mo[a] = -o[a] * rc[a];
+    finite = finite && (fabsf(rc[a]) < __int_as_float(0x7F800000)) && (fabsf(mo[a]) < __int_as_float(0x7F800000));
+  }
+  if (__builtin_amdgcn_ballot_w64(!finite) == 0) {
+    const Uniform U = make_uniform(S);
+    r.t = t_max;
+    return traverse_fast<POLICY>(U, o, dc, rc, mo, t_min, t_max, ratio, stk, r);
+  }
+  return traverse<POLICY>(S, o, d, ratio, stk);
+}
+
 template <int POLICY, int TW, int WAVES, int STACK>
 __global__ __launch_bounds__(64 * WAVES) void trace_kernel(const TraceFrame* __restrict__ frames, uint32_t num_frames,
                                                            uint32_t num_tiles) {
@@ -314,7 +531,7 @@ __global__ __launch_bounds__(64 * WAVES) void trace_kernel(const TraceFrame* __r
     stk.base = s_stack;
     stk.tid = tid;
   }
-  const Hit hit = traverse<POLICY>(S, o, d, F.cam.t_scale_ratio, stk);
+  const Hit hit = trace_ray<POLICY>(S, o, d, F.cam.t_scale_ratio, stk);
 
   const size_t pix = (size_t)y * F.cam.width + x;
   uint32_t tree = 0xFFFFFFFFu, rgba = 0x334D33FFu;
