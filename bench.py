@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Throughput of the serialized-SVO treelet ray caster on one GPU.
 
-Headline (``value``): depth-11 value-noise terrain (2048^3 leaf grid) in
+Headline (``value``, launch shape from the autotuner): depth-11 value-noise terrain (2048^3 leaf grid) in
 4096-node treelets, all resident in HBM; one 1920x1080 FP32 primary-ray frame
 from a seeded camera 0.1 above the surface; treelet (OpenCL renderer)
 traversal policy, full depth (no LOD stop).  A step is one launch that
@@ -229,11 +229,15 @@ def main():
     log(f"hits {hits}/{rays}, mean pushes {res['pushes'].mean():.2f}, algorithmic bytes/launch {alg_bytes}")
 
     frame = svo.Frame.alloc(cam.width, cam.height)
+    view = [svo.View(mm, cam, frame)]
+    renderer = svo.Renderer(0)
+    shape, tune_ms = renderer.autotune(view, args.policy, stream, iters=5)
+    log(f"autotuned launch shape {shape} ({tune_ms:.4f} ms in tuning)")
     if args.warmup > 0:
-        svo.render_timed(mm, [cam], [frame], args.policy, stream, iters=args.warmup)
+        renderer.timed(view, args.policy, stream, iters=args.warmup)
     svo._capi.check(svo.lib.svo_device_synchronize(), "sync")
     w0 = time.perf_counter()
-    ms = svo.render_timed(mm, [cam], [frame], args.policy, stream, iters=args.steps)
+    ms = renderer.timed(view, args.policy, stream, iters=args.steps)
     svo._capi.check(svo.lib.svo_device_synchronize(), "sync")
     wall = time.perf_counter() - w0
     value = rays / (ms / 1e3) / 1e6
@@ -271,6 +275,7 @@ def main():
             "tree_nodes": info["tree_nodes"],
             "hit_rays": hits,
             "mean_pushes": round(float(res["pushes"].mean()), 3),
+            "launch_shape": list(shape),
         },
         "roofline": {
             "bound": "hbm",

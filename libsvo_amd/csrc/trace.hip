@@ -1,35 +1,34 @@
 // Serialized-SVO treelet traversal for gfx950.
 //
-// One ray per lane, one pixel tile per wavefront (16x4 by default), a few
+// One ray per lane, one pixel tile per wavefront (tile_w x 64/tile_w), a few
 // wavefronts per workgroup; every view of a launch may use its own scene.
-// The per-ray stack holds 4 bytes per level, the node index of each parent on
-// the path, in LDS ([level][lane], conflict-free) or in private memory (the
-// launch shape picks; the autotuner measures both).  The child index taken
-// from each parent goes to a 64-bit path register (3 bits per level); the
-// parent node itself is re-read on a pop.
-// A parent's centre, its tmax and its tmin after the pop are recomputed
-// bit-exactly instead of stored:
+// The per-ray stack holds, per level, the node index of the parent left by a
+// push and that parent's tmax (8 bytes), in LDS ([level][lane],
+// conflict-free) or in private memory (the launch shape picks; the autotuner
+// measures both).  The child index taken from each parent goes to a 64-bit
+// path register (3 bits per level); the parent node itself is re-read on a
+// pop (a cache hit: it was fetched a few iterations earlier).
+// Everything else about a parent is recomputed bit-exactly instead of stored:
 //   * centres are dyadic, so parent = child - (-0.5 + bit) * 2^(s - scaleMax)
 //     is exact;
-//   * a node's tmax is the tcMax computed when it was pushed, a pure function
-//     of (ray, centre, size) evaluated with the same FP32 operations;
 //   * after a pop the parent's tmin equals the popped child's tmax, because
 //     the reference sets parentTMin = tcMax right before the push
 //     (opencl/treeletRenderer_rgba.cl:347-357,
 //     src/incore/CpuRaycasterSingleRay3.cpp:355-363) and nothing changes it
-//     until the pop.
+//     until the pop; its tmax is not changed by its own steps either.
 // Every floating-point expression follows the reference's operation order
 // and is compiled with -ffp-contract=off, so the kernel reproduces the CPU
 // restatement bit for bit.
 //
-// The loop is bound by VALU issue (one wave64 VALU instruction per SIMD
-// every few cycles, several waves per SIMD), so its cost is its dynamic
-// instruction count.  Rays whose reciprocal direction is finite on every
-// axis (all but exactly axis-aligned ones) run traverse_fast: plane
-// parameters from the sign of the direction instead of compare-and-swap, two
-// planes per axis per child step, the parent's far faces only on a pop, and a
-// single loop exit (exits inside the loop make the compiler copy the loop
-// state at every exit).  A wave with any other ray runs the generic loop.
+// The loop is bound by VALU issue (several waves per SIMD take turns on one
+// VALU), so its cost is its dynamic instruction count.  Rays whose
+// reciprocal direction is finite on every axis (all but exactly axis-aligned
+// ones) run traverse_fast: plane parameters ordered by the sign of the
+// direction instead of compare-and-swap, two planes per axis per child step,
+// the parent's tmax restored from the stack on a pop, cycle checks scheduled
+// on the wave's (scalar) trip count, and a single loop exit (exits inside the
+// loop make the compiler copy the loop state at every exit).  A wave with any
+// other ray runs the generic loop.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -106,21 +105,32 @@ __device__ __forceinline__ float node_tmax(const float c[3], float h, const floa
   return cl_min(t1[0], cl_min(t1[1], t1[2]));
 }
 
-// Per-ray stack of parent indices.
+// Per-ray stack: per level the node index of the parent left by a push and
+// that parent's tmax (the fast loop restores it on a pop instead of
+// recomputing it; the generic loop uses the index only).
 template <int STACK, int BLOCK>
 struct RayStack;
 template <int BLOCK>
-struct RayStack<SVO_STACK_LDS, BLOCK> {  // LDS, [level][lane]
-  uint32_t* base;
+struct RayStack<SVO_STACK_LDS, BLOCK> {  // LDS, [level][lane] of 8-byte entries
+  uint2* base;
   uint32_t tid;
-  __device__ __forceinline__ void put(int l, uint32_t v) { base[l * BLOCK + tid] = v; }
-  __device__ __forceinline__ uint32_t get(int l) const { return base[l * BLOCK + tid]; }
+  __device__ __forceinline__ void put(int l, uint32_t v) { base[l * BLOCK + tid].x = v; }
+  __device__ __forceinline__ uint32_t get(int l) const { return base[l * BLOCK + tid].x; }
+  __device__ __forceinline__ void put2(int l, uint32_t v, float t) { base[l * BLOCK + tid] = make_uint2(v, __float_as_uint(t)); }
+  __device__ __forceinline__ void get2(int l, uint32_t& v, float& t) const {
+    const uint2 e = base[l * BLOCK + tid];
+    v = e.x;
+    t = __uint_as_float(e.y);
+  }
 };
 template <int BLOCK>
-struct RayStack<SVO_STACK_PRIVATE, BLOCK> {  // private (scratch-backed) array
+struct RayStack<SVO_STACK_PRIVATE, BLOCK> {  // private (scratch-backed) arrays
   uint32_t v[kMaxScale];
+  float tm[kMaxScale];
   __device__ __forceinline__ void put(int l, uint32_t x) { v[l] = x; }
   __device__ __forceinline__ uint32_t get(int l) const { return v[l]; }
+  __device__ __forceinline__ void put2(int l, uint32_t x, float t) { v[l] = x; tm[l] = t; }
+  __device__ __forceinline__ void get2(int l, uint32_t& x, float& t) const { x = v[l]; t = tm[l]; }
 };
 
 // POLICY 0: treeletRenderer_rgba.cl::sample (:201-382).
@@ -348,6 +358,7 @@ __device__ Hit traverse_fast(const Uniform& U, const float o[3], const float d[3
   uint64_t nd = U.nodes[0];
   uint64_t path = 0;
   uint32_t iter = 0;
+  uint32_t trip = 0;  // loop trips of the wave (uniform); schedules the cycle checks
   uint32_t pushes = 0;
   uint64_t snap_path = 0;
   uint32_t snap_iter = 0, snap_scale = 0, snap_tmin = 0, snap_tmax = 0, snap_pushes = 0;
@@ -359,8 +370,11 @@ __device__ Hit traverse_fast(const Uniform& U, const float o[3], const float d[3
   do {
     const float se = exp2i(scale - U.smax);  // half-edge of the current node
     ++iter;
-    if (iter >= 128u) {
-      if ((iter & (iter - 1u)) == 0u) {
+    ++trip;
+    // snapshots at power-of-two trips >= 128, comparisons every 8th trip: a
+    // repeat found at any multiple of the period is a period too
+    if (trip >= 128u && (trip & 7u) == 0u) {
+      if ((trip & (trip - 1u)) == 0u) {
         snap_iter = iter; snap_scale = (uint32_t)scale; snap_path = path;
         snap_tmin = __float_as_uint(tmin); snap_tmax = __float_as_uint(tmax); snap_pushes = pushes;
       } else if (snap_iter != 0u && (uint32_t)scale == snap_scale && path == snap_path &&
@@ -414,7 +428,7 @@ __device__ Hit traverse_fast(const Uniform& U, const float o[3], const float d[3
           outcome = 4;
         } else {
           ++pushes;
-          stk.put(scale - U.mps, nidx);
+          stk.put2(scale - U.mps, nidx, tmax);
           path = (path << 3) | ci;
           --scale;
           const float h = se * 0.5f;
@@ -432,19 +446,18 @@ __device__ Hit traverse_fast(const Uniform& U, const float o[3], const float d[3
       outcome = 2;  // left the root
     } else {
       ++scale;
-      fidx = stk.get(scale - U.mps);
+      float tparent;
+      stk.get2(scale - U.mps, fidx, tparent);
       fetch = true;
-      const uint32_t ci = (uint32_t)(path & 7u);
+      const uint32_t pl = (uint32_t)path;
       path >>= 3;
-      const float sp = se * 2.0f;  // half-edge of the parent
-      float tfar[3];
-#pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        c[a] = c[a] + (((ci >> (2 - a)) & 1u) ? -se : se);
-        tfar[a] = rc[a] * (c[a] + (neg[a] ? -sp : sp)) + mo[a];
-      }
+      const uint32_t seb = __float_as_uint(se);
+      // parent centre = child centre -/+ se (bit set: the child was the upper half)
+      c[0] = c[0] + __uint_as_float(seb | ((pl << 29) & 0x80000000u));
+      c[1] = c[1] + __uint_as_float(seb | ((pl << 30) & 0x80000000u));
+      c[2] = c[2] + __uint_as_float(seb | (pl << 31));
       tmin = tmax;
-      tmax = (scale == U.smax - 1) ? t_max : cl_min(tfar[0], cl_min(tfar[1], tfar[2]));
+      tmax = tparent;
     }
     if (fetch) {
       nidx = fidx;
@@ -528,7 +541,7 @@ __global__ __launch_bounds__(64 * WAVES) void trace_kernel(const TraceFrame* __r
   pixel_ray(F.cam, x, y, d);
   RayStack<STACK, kBlock> stk;
   if constexpr (STACK == SVO_STACK_LDS) {
-    stk.base = s_stack;
+    stk.base = reinterpret_cast<uint2*>(s_stack);
     stk.tid = tid;
   }
   const Hit hit = trace_ray<POLICY>(S, o, d, F.cam.t_scale_ratio, stk);
@@ -613,7 +626,7 @@ int launch_trace(const LaunchShape& shape, const TraceFrame* frames_dev, uint32_
   const TraceFn fn = policy == SVO_POLICY_CPU3 ? pick_tile<2>(shape) : pick_tile<0>(shape);
   const uint32_t block = 64u * shape.waves;
   const dim3 grid((num_tiles + shape.waves - 1) / shape.waves);
-  const size_t lds = shape.stack == SVO_STACK_LDS ? (size_t)stack_levels * block * sizeof(uint32_t) : 0;
+  const size_t lds = shape.stack == SVO_STACK_LDS ? (size_t)stack_levels * block * sizeof(uint2) : 0;
   hipLaunchKernelGGL(fn, grid, dim3(block), lds, (hipStream_t)stream, frames_dev, num_frames, num_tiles);
   return (int)hipGetLastError();
 }
