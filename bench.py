@@ -38,6 +38,27 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+PROFILE = os.path.join(ROOT, "profiles", "trace_kernel_terrain_d11_1080p.md")
+
+
+def profiled_traffic(shape):
+    """HBM bytes per launch of the 1080p depth-11 frame for a launch shape,
+    from the FETCH_SIZE / WRITE_SIZE counter passes summarised in the
+    committed profile (None when the shape has no row there)."""
+    import re
+    try:
+        text = open(PROFILE).read()
+    except OSError:
+        return None
+    key = f"| ({shape[0]}, {shape[1]}, {shape[2]}) |"
+    for line in text.splitlines():
+        if line.startswith(key):
+            m = re.findall(r"\|\s*([\d,]+)\s*", line[len(key) - 1:])
+            if len(m) >= 3:
+                return int(m[2].replace(",", ""))
+    return None
+
+
 def cpu_baseline(scene, cam, threads, frames=5):
     """Oracle restatement of CpuRaycasterSingleRay3 (the reference CPU caster)
     timed on the host: OpenMP over rows on `threads` threads, median of
@@ -283,7 +304,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5),
-            "traffic": None,
+            "traffic": profiled_traffic(shape),
         },
     }
     extra_scenes = {"terrain11": (scene, mm)}
