@@ -12,6 +12,10 @@ Also reported in the same line:
 * ``batched``: 64 views over the depth-11 terrain and a depth-10 sphere
   cloud, resolutions uniform over {256^2, 512^2, 720p, 1080p}, one launch
   captured in a hipGraph, launch shape from the autotuner;
+* ``ooc``: depth-13 terrain in 16384-node treelets (larger than the pool)
+  behind an 8 GiB device pool, 300 1080p frames along a seeded fly-through;
+  each frame runs the analyse pass, the feedback update (page-ins,
+  evictions), the upload and the render: frames/s and page-ins per frame;
 * ``serving``: Poisson request streams over four resident scenes (Menger
   depth 9, terrain depth 11, sphere cloud depth 10, terrain depth 12) in a
   64 GB HBM budget, served by the continuous-batching server and by the plain
@@ -153,6 +157,75 @@ def batched_section(cache, stream, policy, launches=10):
     }
 
 
+def ooc_section(policy, frames, depth=13, treelet_bytes=131072, pool_bytes=8 << 30, divide=8):
+    """Out-of-core fly-through: analyse -> feedback update -> upload -> render
+    per frame, starting from the root treelet alone."""
+    import numpy as np
+    import libsvo_amd as svo
+    from libsvo_amd import workloads
+    t0 = time.perf_counter()
+    sc = workloads.terrain_scene(depth=depth, treelet_bytes=treelet_bytes)
+    info = sc.info
+    N = info["nodes_per_treelet"]
+    slots = pool_bytes // (N * 16)  # nodes (8 B) + colour (4 B) + normal (4 B) per node
+    mm = svo.TreeletMemoryManagerHip(sc, slots * info["treelet_bytes"], device=0)
+    log(f"ooc: depth-{depth} terrain built in {time.perf_counter() - t0:.1f}s: {info['num_treelets']} treelets "
+        f"({info['num_treelets'] * treelet_bytes / 2**30:.1f} GiB of nodes), pool {slots} slots "
+        f"({mm.info['device_bytes'] / 2**30:.2f} GiB in HBM)")
+    cams = workloads.flythrough_cameras(frames, lod=1.0)
+    rng = np.random.default_rng(workloads.FLIGHT_SEED)
+    frame = svo.Frame.alloc(cams[0].width, cams[0].height)
+    r = svo.Renderer(0)
+    s = svo.Stream()
+    view = [svo.View(mm, cams[0], frame)]
+    pages, evictions, upload_bytes, failed = [], [], [], 0
+    t_feedback = t_upload = t_render = 0.0
+    wall0 = time.perf_counter()
+    for i, cam in enumerate(cams):
+        a = time.perf_counter()
+        jit = tuple(float(v) for v in rng.uniform(-1.0, 1.0, 2))
+        mm.performAnalysePass(cam, divide, jit, policy, s)
+        st = mm.updateClientSideIncoreBuffer(s)
+        b = time.perf_counter()
+        up = mm.updateDeviceMemory(s)
+        c = time.perf_counter()
+        view[0].camera = cam
+        r.render(view, policy, s)
+        s.synchronize()
+        d = time.perf_counter()
+        t_feedback += b - a
+        t_upload += c - b
+        t_render += d - c
+        pages.append(st["inserted"])
+        evictions.append(st["evicted"])
+        failed += st["insert_failed"]
+        upload_bytes.append(up["bytes"])
+        if i % 50 == 0:
+            log(f"ooc frame {i}: {st['inserted']} page-ins, {st['evicted']} evictions, "
+                f"{mm.info['resident_treelets']} resident, {up['bytes'] / 2**20:.1f} MiB uploaded")
+    wall = time.perf_counter() - wall0
+    res = {
+        "workload": f"terrain depth {depth}, {N}-node treelets ({info['num_treelets']} treelets, "
+                    f"{info['num_treelets'] * treelet_bytes / 2**30:.1f} GiB of nodes), {slots}-slot pool "
+                    f"({mm.info['device_bytes'] / 2**30:.2f} GiB HBM), {frames} frames "
+                    f"{cams[0].width}x{cams[0].height} along a seeded Catmull-Rom fly-through, LOD 1 pixel, "
+                    f"analyse pass 1/{divide} per axis, starting from the root treelet",
+        "frames": frames, "frames_per_s": round(frames / wall, 1),
+        "ms_per_frame": round(wall * 1e3 / frames, 3),
+        "ms_feedback": round(t_feedback * 1e3 / frames, 3), "ms_upload": round(t_upload * 1e3 / frames, 3),
+        "ms_render": round(t_render * 1e3 / frames, 3),
+        "page_ins_per_frame": round(float(np.mean(pages)), 2), "page_ins_max": int(max(pages)),
+        "page_ins_after_frame_30": round(float(np.mean(pages[30:])), 2) if frames > 30 else None,
+        "evictions_total": int(sum(evictions)), "insert_failed_total": failed,
+        "upload_mib_per_frame": round(float(np.mean(upload_bytes)) / 2**20, 2),
+        "resident_treelets_end": mm.info["resident_treelets"], "seed": workloads.FLIGHT_SEED,
+    }
+    log(f"ooc: {res['frames_per_s']} frames/s, {res['page_ins_per_frame']} page-ins/frame, "
+        f"{res['upload_mib_per_frame']} MiB/frame uploaded, render {res['ms_render']} ms, "
+        f"feedback {res['ms_feedback']} ms, upload {res['ms_upload']} ms")
+    return res
+
+
 def serving_section(cache, rates, seconds, policy):
     """Poisson request streams over four resident scenes: continuous batching
     against the plain one-frame-per-launch loop."""
@@ -212,6 +285,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-ooc", action="store_true")
+    ap.add_argument("--ooc-frames", type=int, default=300)
     ap.add_argument("--serving-seconds", type=float, default=3.0, help="length of each request stream")
     ap.add_argument("--serving-rates", default="200,1000,2000", help="Poisson arrival rates (req/s)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
@@ -310,6 +385,8 @@ def main():
     extra_scenes = {"terrain11": (scene, mm)}
     if not args.no_batched:
         out["batched"] = batched_section(extra_scenes, stream, args.policy)
+    if not args.no_ooc:
+        out["ooc"] = ooc_section(args.policy, args.ooc_frames)
     if not args.no_serving:
         rates = [float(r) for r in args.serving_rates.split(",") if r]
         out["serving"] = serving_section(extra_scenes, rates, args.serving_seconds, args.policy)

@@ -304,6 +304,10 @@ typedef struct svo_view {
   svo_mm* mm;
   svo_camera camera;
   svo_frame_out out;
+  /* > 0: pixels with x % feedback_stride == 0 and y % feedback_stride == 0
+   * also record analysis feedback into mm's tables (see svo_mm_analyse) */
+  uint32_t feedback_stride;
+  uint32_t reserved;
 } svo_view;
 
 /* Launch shape chosen by svo_autotune (or set by hand):
@@ -359,6 +363,66 @@ svo_status svo_batch_set_cameras(svo_batch* b, const svo_camera* cams,
                                  uint32_t n, void* stream);
 svo_status svo_batch_launch(svo_batch* b, void* stream);
 void svo_batch_destroy(svo_batch* b);
+
+/* ---------------------------------------------- analysis feedback (OOC) */
+
+/* The analyse pass of the OpenCL app (RenderPassAnalyse::performAnalysePass,
+ * src/RenderPassAnalyse.cpp:149-369, kernel renderToFeedbackBuffer +
+ * sampleAnalyse, opencl/treeletRenderer_rgba.cl:773-1011), fused into the
+ * traversal: every ray of a W/divide x H/divide frame over the same frustum
+ * (far lower-left corner moved by jitter_x, jitter_y in [-1, 1] half analysis
+ * pixels, the reference's "rumble") traces with the given policy and records
+ * its hit on the device:
+ *   - leaf hit: error = scale_exp2 - tScaleRatio*tcMin; when the leaf's
+ *     first-child field holds a sub-treelet gid, that treelet is asked for;
+ *   - inner node at the LOD / depth limit: error = (tScaleRatio*tcMin >
+ *     scale_exp2);
+ *   - miss or maxLoops: no record.
+ * The treelet holding each hit node is marked visible with the largest
+ * error, each asked treelet keeps its largest error (device tables, no
+ * per-pixel readback).  Views with feedback_stride > 0 record the same way
+ * during an ordinary render. */
+svo_status svo_mm_analyse(svo_mm* mm, const svo_camera* cam, uint32_t divide,
+                          float jitter_x, float jitter_y, int policy,
+                          void* stream);
+
+/* The analysis frame svo_mm_analyse traces: width/divide x height/divide
+ * pixels spanning the same far plane, the far lower-left corner moved by
+ * (jitter_x * pixel_h + jitter_y * pixel_v) * divide / 2
+ * (RenderPassAnalyse.cpp:172-184; FP32, in that operation order). */
+svo_status svo_analysis_camera(const svo_camera* cam, uint32_t divide,
+                               float jitter_x, float jitter_y,
+                               svo_camera* out);
+
+typedef struct svo_feedback_stats {
+  uint32_t visible_slots;      /* slots holding a recorded hit             */
+  uint32_t visible_treelets;   /* those treelets and their ancestors       */
+  uint32_t requests;           /* distinct treelets asked for              */
+  uint32_t requests_kept;      /* after the 12 MiB cap, largest error first */
+  uint32_t inserted;           /* page-ins                                 */
+  uint32_t insert_failed;      /* no free or freeable slot                 */
+  uint32_t evicted;            /* treelets freed by freeIncorePosition     */
+  uint32_t resident_treelets;
+} svo_feedback_stats;
+
+/* updateClientSideIncoreBuffer(RenderPassAnalyse*)
+ * (src/TreeletMemoryManager.cpp:392-443) from the records gathered since the
+ * previous call: reads the tables back on `stream` (the recording launches
+ * must be ordered before it on that stream), marks visible treelets and their
+ * ancestors, ages every other slot, and inserts the asked treelets in
+ * descending error (at most 12 MiB of treelets), evicting invisible ones as
+ * needed.  Follow with svo_mm_update_device to upload the changed slots. */
+svo_status svo_mm_apply_feedback(svo_mm* mm, void* stream,
+                                 svo_feedback_stats* stats);
+
+/* The same update from caller-supplied aggregated records (no device
+ * access; works on host-only handles): slots[i] was hit with largest error
+ * slot_errors[i]; gids[i] (distinct, non-root) asked for with largest error
+ * gid_errors[i]. */
+svo_status svo_mm_apply_records(svo_mm* mm, const uint32_t* slots,
+                                const float* slot_errors, uint32_t n_slots,
+                                const uint32_t* gids, const float* gid_errors,
+                                uint32_t n_gids, svo_feedback_stats* stats);
 
 /* ------------------------------------------------------------ serving */
 

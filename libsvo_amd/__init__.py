@@ -247,6 +247,34 @@ class TreeletMemoryManagerHip:
     def getNumNodesPerTreelet(self) -> int:
         return self.info["nodes_per_treelet"]
 
+    def performAnalysePass(self, cam: svo_camera, divide: int = 8, jitter=(0.0, 0.0), policy="treelet",
+                           stream: Optional["Stream"] = None) -> None:
+        """Analyse pass over a (W/divide) x (H/divide) frame of `cam`; its
+        records accumulate on the device until updateClientSideIncoreBuffer."""
+        check(lib.svo_mm_analyse(self._h, ctypes.byref(cam), int(divide), float(jitter[0]), float(jitter[1]),
+                                 _policy(policy), stream.handle if stream else None), "svo_mm_analyse")
+
+    def updateClientSideIncoreBuffer(self, stream: Optional["Stream"] = None) -> Dict[str, int]:
+        """Applies the recorded feedback: visibility update and page-ins
+        (call updateDeviceMemory afterwards)."""
+        st = _capi.svo_feedback_stats()
+        check(lib.svo_mm_apply_feedback(self._h, stream.handle if stream else None, ctypes.byref(st)),
+              "svo_mm_apply_feedback")
+        return {k: getattr(st, k) for k, _ in _capi.svo_feedback_stats._fields_}
+
+    def applyRecords(self, slots, slot_errors, gids, gid_errors) -> Dict[str, int]:
+        """The same update from aggregated records given by the caller."""
+        s = np.ascontiguousarray(slots, np.uint32)
+        se = np.ascontiguousarray(slot_errors, np.float32)
+        g = np.ascontiguousarray(gids, np.uint32)
+        ge = np.ascontiguousarray(gid_errors, np.float32)
+        if s.size != se.size or g.size != ge.size:
+            raise ValueError("one error per slot and per gid")
+        st = _capi.svo_feedback_stats()
+        check(lib.svo_mm_apply_records(self._h, s.ctypes.data, se.ctypes.data, s.size, g.ctypes.data, ge.ctypes.data,
+                                       g.size, ctypes.byref(st)), "svo_mm_apply_records")
+        return {k: getattr(st, k) for k, _ in _capi.svo_feedback_stats._fields_}
+
     def getSlots(self) -> List[int]:
         out = []
         g = ctypes.c_int32()
@@ -347,6 +375,14 @@ def render(mm: TreeletMemoryManagerHip, cam: svo_camera, policy="treelet", pushe
     return f.fetch()
 
 
+def analysis_camera(cam: svo_camera, divide: int = 8, jitter=(0.0, 0.0)) -> svo_camera:
+    """The reduced frame an analyse pass traces for `cam`."""
+    out = svo_camera()
+    check(lib.svo_analysis_camera(ctypes.byref(cam), int(divide), float(jitter[0]), float(jitter[1]),
+                                  ctypes.byref(out)), "svo_analysis_camera")
+    return out
+
+
 def ray_dirs(cam: svo_camera) -> np.ndarray:
     buf = DeviceBuffer(12 * cam.width * cam.height)
     check(lib.svo_debug_ray_dirs(ctypes.byref(cam), buf.ptr, None), "svo_debug_ray_dirs")
@@ -364,13 +400,17 @@ LAUNCH_SHAPES = [(t, w, s) for t in (8, 16, 32) for w in (1, 2, 4) for s in ("ld
 
 @dataclass
 class View:
-    """One camera of one resident scene rendered into one frame."""
+    """One camera of one resident scene rendered into one frame (frame None:
+    no outputs, e.g. a feedback-only view).  feedback_stride > 0 records
+    analysis feedback for every feedback_stride-th pixel in x and y."""
     mm: TreeletMemoryManagerHip
     camera: svo_camera
-    frame: Frame
+    frame: Optional[Frame]
+    feedback_stride: int = 0
 
     def c_view(self) -> svo_view:
-        return svo_view(self.mm.handle, self.camera, self.frame.out())
+        out = self.frame.out() if self.frame is not None else svo_frame_out()
+        return svo_view(self.mm.handle, self.camera, out, int(self.feedback_stride), 0)
 
 
 def _view_array(views: Sequence[View]):

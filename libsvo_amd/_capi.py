@@ -61,7 +61,13 @@ class svo_frame_out(ctypes.Structure):
 
 
 class svo_view(ctypes.Structure):
-    _fields_ = [("mm", vp), ("camera", svo_camera), ("out", svo_frame_out)]
+    _fields_ = [("mm", vp), ("camera", svo_camera), ("out", svo_frame_out), ("feedback_stride", u32),
+                ("reserved", u32)]
+
+
+class svo_feedback_stats(ctypes.Structure):
+    _fields_ = [(n, u32) for n in ("visible_slots", "visible_treelets", "requests", "requests_kept", "inserted",
+                                    "insert_failed", "evicted", "resident_treelets")]
 
 
 class svo_server_config(ctypes.Structure):
@@ -128,6 +134,10 @@ SIGNATURES = {
     "svo_batch_set_cameras": (ctypes.c_int, [vp, P(svo_camera), u32, vp]),
     "svo_batch_launch": (ctypes.c_int, [vp, vp]),
     "svo_batch_destroy": (None, [vp]),
+    "svo_analysis_camera": (ctypes.c_int, [P(svo_camera), u32, f32, f32, P(svo_camera)]),
+    "svo_mm_analyse": (ctypes.c_int, [vp, P(svo_camera), u32, f32, f32, ctypes.c_int, vp]),
+    "svo_mm_apply_feedback": (ctypes.c_int, [vp, vp, P(svo_feedback_stats)]),
+    "svo_mm_apply_records": (ctypes.c_int, [vp, vp, vp, u32, vp, vp, u32, P(svo_feedback_stats)]),
     "svo_server_create": (ctypes.c_int, [vp, P(svo_server_config), P(vp)]),
     "svo_server_destroy": (None, [vp]),
     "svo_server_add_scene": (ctypes.c_int, [vp, vp, P(u32)]),

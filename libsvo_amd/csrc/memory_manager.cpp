@@ -15,6 +15,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <queue>
 #include <set>
@@ -67,6 +68,18 @@ struct svo_mm {
   uint32_t* d_normals = nullptr;
   int32_t* d_slot_gid = nullptr;
   svo_renderer* renderer = nullptr;  // default renderer of svo_render
+  // analysis feedback tables (see TracePools) and their pinned readback
+  unsigned long long* d_fb_slot = nullptr;
+  unsigned long long* d_fb_gid = nullptr;
+  unsigned long long* d_fb_out = nullptr;
+  uint32_t* d_fb_queue = nullptr;
+  uint32_t* d_fb_count = nullptr;
+  unsigned long long* h_fb_slot = nullptr;
+  unsigned long long* h_fb_out = nullptr;
+  uint32_t* h_fb_queue = nullptr;
+  uint32_t* h_fb_count = nullptr;
+  uint32_t fb_stamp = 1;
+  uint64_t evictions = 0;  // treelets freed by freeIncorePosition (incl. descendants)
 
   bool host_only() const { return device < 0; }
 
@@ -81,6 +94,15 @@ struct svo_mm {
     hipFree(d_colors);
     hipFree(d_normals);
     hipFree(d_slot_gid);
+    hipFree(d_fb_slot);
+    hipFree(d_fb_gid);
+    hipFree(d_fb_out);
+    hipFree(d_fb_queue);
+    hipFree(d_fb_count);
+    if (h_fb_slot) hipHostFree(h_fb_slot);
+    if (h_fb_out) hipHostFree(h_fb_out);
+    if (h_fb_queue) hipHostFree(h_fb_queue);
+    if (h_fb_count) hipHostFree(h_fb_count);
     hipSetDevice(cur);
   }
 
@@ -137,6 +159,7 @@ struct svo_mm {
       treelet_slot[c] = -1;
       slots[s] = SlotInfo();
       --resident;
+      ++evictions;
     }
   }
 
@@ -162,7 +185,64 @@ struct svo_mm {
     if (freeable == 0) return false;
     free_descendants(freeable);
     remove_one(freeable);
+    ++evictions;
     return true;
+  }
+
+  // The host half of the analyse pass and the client-side update, from
+  // aggregated records: `seen` holds (slot, largest error) of the slots that
+  // hold a record's node, `asked` (gid, largest error) of the sub-treelets
+  // that records ask for, each gid once.
+  //  * RenderPassAnalyse::performAnalysePass (src/RenderPassAnalyse.cpp:
+  //    248-356): every seen treelet and its ancestors are visible ("old"),
+  //    the root with error 99; the asked treelets ("new") are ordered by
+  //    descending error (ties in ascending gid, the order in which the
+  //    reference's map feeds its multiset) and capped at 12 MiB of treelets;
+  //  * updateClientSideIncoreBuffer (src/TreeletMemoryManager.cpp:392-443):
+  //    every slot but the root's loses one visibility, visible treelets get
+  //    MAX_VISIBILITY, then the new treelets are inserted in order
+  //    (insertTreeletIntoIncoreBuffer, evicting through freeIncorePosition).
+  void apply_records(const std::vector<std::pair<uint32_t, float>>& seen,
+                     std::vector<std::pair<uint32_t, float>> asked, svo_feedback_stats* st) {
+    svo_feedback_stats S{};
+    std::map<uint32_t, float> old;
+    for (const auto& e : seen) {
+      if (e.first >= num_slots || slots[e.first].gid < 0) continue;
+      ++S.visible_slots;
+      uint32_t g = (uint32_t)slots[e.first].gid;
+      while (g != 0) {
+        auto it = old.find(g);
+        if (it == old.end()) old[g] = e.second;
+        else if (e.second > it->second) it->second = e.second;
+        g = scene->treelets[g].h.parent_gid;
+      }
+      old[0] = 99.0f;
+    }
+    S.visible_treelets = (uint32_t)old.size();
+    S.requests = (uint32_t)asked.size();
+    std::stable_sort(asked.begin(), asked.end(), [](const std::pair<uint32_t, float>& a, const std::pair<uint32_t, float>& b) {
+      return a.first < b.first;
+    });
+    std::stable_sort(asked.begin(), asked.end(), [](const std::pair<uint32_t, float>& a, const std::pair<uint32_t, float>& b) {
+      return a.second > b.second;
+    });
+    const uint32_t cap = (uint32_t)((12.0 * 1048576.0f) / (double)scene->treelet_bytes);
+    if (asked.size() > cap) asked.resize(cap);
+    S.requests_kept = (uint32_t)asked.size();
+    for (uint32_t i = 1; i < num_slots; ++i) --slots[i].visibility;
+    for (const auto& e : old) {
+      const int32_t sl = treelet_slot[e.first];
+      if (sl >= 0) slots[sl].visibility = kMaxVisibility;
+    }
+    const uint64_t ev0 = evictions;
+    for (const auto& e : asked) {
+      if (insert(e.first, e.second) == SVO_OK) ++S.inserted;
+      else ++S.insert_failed;
+    }
+    clear_error();
+    S.evicted = (uint32_t)(evictions - ev0);
+    S.resident_treelets = resident;
+    if (st) *st = S;
   }
 
   // insertTreeletIntoIncoreBuffer (src/TreeletMemoryManager.cpp:455-548).
@@ -305,6 +385,18 @@ extern "C" svo_status svo_mm_create(const svo_scene* scene, uint64_t incore_byte
     set_error("svo_mm_create: hipMalloc of %llu B of pools failed", (unsigned long long)(pool_nodes * 16 + slots * 4));
     return SVO_ERR_NOMEM;
   }
+  const size_t ng = scene->treelets.size();
+  if (hipMalloc(&mm->d_fb_slot, slots * 8) != hipSuccess || hipMalloc(&mm->d_fb_gid, ng * 8) != hipSuccess ||
+      hipMalloc(&mm->d_fb_out, ng * 8) != hipSuccess || hipMalloc(&mm->d_fb_queue, ng * 4) != hipSuccess ||
+      hipMalloc(&mm->d_fb_count, 4) != hipSuccess || hipHostMalloc(&mm->h_fb_slot, slots * 8, 0) != hipSuccess ||
+      hipHostMalloc(&mm->h_fb_out, ng * 8, 0) != hipSuccess || hipHostMalloc(&mm->h_fb_queue, ng * 4, 0) != hipSuccess ||
+      hipHostMalloc(&mm->h_fb_count, 4, 0) != hipSuccess) {
+    set_error("svo_mm_create: feedback tables for %zu treelets could not be allocated", ng);
+    return SVO_ERR_NOMEM;
+  }
+  HIP_TRY(hipMemset(mm->d_fb_slot, 0, slots * 8));
+  HIP_TRY(hipMemset(mm->d_fb_gid, 0, ng * 8));
+  HIP_TRY(hipMemset(mm->d_fb_count, 0, 4));
   // initClBuffer: the whole incore buffer goes up once
   r = mm->upload_range(0, mm->num_slots, nullptr, nullptr);
   if (r != SVO_OK) return r;
@@ -363,7 +455,9 @@ extern "C" svo_status svo_mm_get_info(const svo_mm* mm, svo_mm_info* out) {
   out->resident_treelets = mm->resident;
   out->device = (uint32_t)mm->device;
   out->pool_nodes = (uint64_t)mm->num_slots * mm->N;
-  out->device_bytes = mm->host_only() ? 0 : out->pool_nodes * 16 + (uint64_t)mm->num_slots * 4;
+  out->device_bytes = mm->host_only() ? 0
+                                      : out->pool_nodes * 16 + (uint64_t)mm->num_slots * 12 +
+                                            (uint64_t)mm->scene->treelets.size() * 20 + 4;
   return SVO_OK;
 }
 
@@ -436,6 +530,13 @@ svo_status mm_trace_pools(const svo_mm* mm, int policy, TracePools* P) {
   P->slot_shift = ((mm->N & (mm->N - 1)) == 0) ? (uint32_t)__builtin_ctz(mm->N) : 32u;
   P->num_slots = mm->num_slots;
   P->pool_nodes = mm->num_slots * mm->N;
+  P->num_gids = (uint32_t)mm->scene->treelets.size();
+  P->fb_slot = mm->d_fb_slot;
+  P->fb_gid = mm->d_fb_gid;
+  P->fb_queue = mm->d_fb_queue;
+  P->fb_count = mm->d_fb_count;
+  P->fb_cap = P->num_gids;
+  P->fb_stamp = mm->fb_stamp;
   if (policy == SVO_POLICY_TREELET) {
     // MAX_STACK_SIZE 13 / MAX_SVO_RAYCAST_DEPTH 12 (treeletRenderer_rgba.cl:7-8),
     // widened to max(13, D+1) so that leaves deeper than 12 are reachable.
@@ -490,6 +591,106 @@ extern "C" svo_status svo_render_timed(svo_mm* mm, const svo_camera* cams, uint3
   if (!r) return SVO_ERR_HIP;
   const std::vector<svo_view> v = make_views(mm, cams, n_cams, outs);
   return svo_render_views_timed(r, v.data(), n_cams, policy, stream, iters, mean_ms);
+}
+
+// ------------------------------------------------------------ analysis feedback
+
+namespace {
+float key_error(unsigned long long key) {
+  const uint32_t k = (uint32_t)key;
+  const uint32_t b = (k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k;
+  float f;
+  std::memcpy(&f, &b, 4);
+  return f;
+}
+}  // namespace
+
+extern "C" svo_status svo_analysis_camera(const svo_camera* cam, uint32_t divide, float jitter_x, float jitter_y,
+                                          svo_camera* out) {
+  if (!cam || !out) { set_error("svo_analysis_camera: null argument"); return SVO_ERR_ARG; }
+  if (divide == 0 || cam->width < divide || cam->height < divide) {
+    set_error("svo_analysis_camera: divide %u does not fit a %ux%u frame", divide, cam->width, cam->height);
+    return SVO_ERR_ARG;
+  }
+  if (!(jitter_x >= -1.0f && jitter_x <= 1.0f && jitter_y >= -1.0f && jitter_y <= 1.0f)) {
+    set_error("svo_analysis_camera: jitter must lie in [-1, 1]");
+    return SVO_ERR_ARG;
+  }
+  svo_camera c = *cam;
+  c.width = cam->width / divide;
+  c.height = cam->height / divide;
+  const float sx = (float)cam->width / (float)c.width, sy = (float)cam->height / (float)c.height;
+  for (int a = 0; a < 3; ++a) {
+    c.pixel_h[a] = cam->pixel_h[a] * sx;
+    c.pixel_v[a] = cam->pixel_v[a] * sy;
+    c.far_lower_left[a] = (cam->far_lower_left[a] + ((jitter_x * (float)divide) * cam->pixel_h[a]) * 0.5f) +
+                          ((jitter_y * (float)divide) * cam->pixel_v[a]) * 0.5f;
+  }
+  *out = c;
+  return SVO_OK;
+}
+
+extern "C" svo_status svo_mm_analyse(svo_mm* mm, const svo_camera* cam, uint32_t divide, float jitter_x,
+                                     float jitter_y, int policy, void* stream) {
+  if (!mm || !cam) { set_error("svo_mm_analyse: null argument"); return SVO_ERR_ARG; }
+  if (mm->host_only()) { set_error("svo_mm_analyse: handle has no device pools"); return SVO_ERR_STATE; }
+  svo_view v{};
+  v.mm = mm;
+  svo_status r = svo_analysis_camera(cam, divide, jitter_x, jitter_y, &v.camera);
+  if (r != SVO_OK) return r;
+  v.out = svo_frame_out{nullptr, nullptr, nullptr, nullptr, nullptr};
+  v.feedback_stride = 1;
+  svo_renderer* rd = mm_default_renderer(mm);
+  if (!rd) return SVO_ERR_HIP;
+  return svo_render_views(rd, &v, 1, policy, stream);
+}
+
+extern "C" svo_status svo_mm_apply_records(svo_mm* mm, const uint32_t* slots, const float* slot_errors,
+                                           uint32_t n_slots, const uint32_t* gids, const float* gid_errors,
+                                           uint32_t n_gids, svo_feedback_stats* stats) {
+  if (!mm || (n_slots && (!slots || !slot_errors)) || (n_gids && (!gids || !gid_errors))) {
+    set_error("svo_mm_apply_records: null argument");
+    return SVO_ERR_ARG;
+  }
+  std::vector<std::pair<uint32_t, float>> seen(n_slots), asked;
+  for (uint32_t i = 0; i < n_slots; ++i) seen[i] = {slots[i], slot_errors[i]};
+  std::set<uint32_t> dup;
+  for (uint32_t i = 0; i < n_gids; ++i) {
+    if (gids[i] == 0 || gids[i] >= mm->scene->treelets.size()) { set_error("svo_mm_apply_records: gid %u out of range", gids[i]); return SVO_ERR_ARG; }
+    if (!dup.insert(gids[i]).second) { set_error("svo_mm_apply_records: gid %u listed twice", gids[i]); return SVO_ERR_ARG; }
+    asked.push_back({gids[i], gid_errors[i]});
+  }
+  mm->apply_records(seen, asked, stats);
+  return SVO_OK;
+}
+
+extern "C" svo_status svo_mm_apply_feedback(svo_mm* mm, void* stream, svo_feedback_stats* stats) {
+  if (!mm) { set_error("svo_mm_apply_feedback: null handle"); return SVO_ERR_ARG; }
+  if (mm->host_only()) { set_error("svo_mm_apply_feedback: handle has no device pools"); return SVO_ERR_STATE; }
+  HIP_TRY(hipSetDevice(mm->device));
+  hipStream_t st = (hipStream_t)stream;
+  const uint32_t ng = (uint32_t)mm->scene->treelets.size();
+  HIP_TRY(hipMemcpyAsync(mm->h_fb_slot, mm->d_fb_slot, (size_t)mm->num_slots * 8, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(mm->h_fb_count, mm->d_fb_count, 4, hipMemcpyDeviceToHost, st));
+  int e = launch_feedback_gather(mm->d_fb_gid, mm->d_fb_queue, mm->d_fb_count, ng, mm->d_fb_out, st);
+  if (e != 0) { set_error("feedback gather launch failed: %s", hipGetErrorString((hipError_t)e)); return SVO_ERR_HIP; }
+  HIP_TRY(hipStreamSynchronize(st));
+  const uint32_t n = std::min(*mm->h_fb_count, ng);
+  if (n) {
+    HIP_TRY(hipMemcpyAsync(mm->h_fb_queue, mm->d_fb_queue, (size_t)n * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(mm->h_fb_out, mm->d_fb_out, (size_t)n * 8, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+  }
+  const uint32_t stamp = mm->fb_stamp;
+  std::vector<std::pair<uint32_t, float>> seen, asked;
+  for (uint32_t s = 0; s < mm->num_slots; ++s) {
+    const unsigned long long k = mm->h_fb_slot[s];
+    if ((uint32_t)(k >> 32) == stamp) seen.push_back({s, key_error(k)});
+  }
+  for (uint32_t i = 0; i < n; ++i) asked.push_back({mm->h_fb_queue[i], key_error(mm->h_fb_out[i])});
+  ++mm->fb_stamp;  // records of this cycle are now stale
+  mm->apply_records(seen, asked, stats);
+  return SVO_OK;
 }
 
 // ------------------------------------------------------------ device utilities

@@ -55,7 +55,9 @@ svo_status stage_views(const svo_view* views, uint32_t n, int policy, const Laun
     F.out = v.out;
     F.tile_begin = (uint32_t)tiles;
     F.tiles_x = (c.width + tw - 1) / tw;
-    F.pad[0] = F.pad[1] = 0;
+    if (v.feedback_stride > 65535u) { set_error("view %u: feedback stride %u too large", i, v.feedback_stride); return SVO_ERR_ARG; }
+    F.fb_stride = v.feedback_stride;
+    F.pad = 0;
     tiles += (uint64_t)F.tiles_x * ((c.height + th - 1) / th);
     levels = std::max(levels, F.pools.stack_levels);
   }

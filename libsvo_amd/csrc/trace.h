@@ -25,7 +25,17 @@ struct TracePools {
   uint32_t lod_cap;         // policy-A depth cap (MAX_SVO_RAYCAST_DEPTH), 0 = none
   uint32_t min_push_scale;  // lowest scale a push can start from
   uint32_t stack_levels;    // scale_max - min_push_scale (stack depth)
-  uint32_t pad;
+  uint32_t num_gids;        // treelets in the scene (size of fb_gid)
+  // Analysis feedback tables of the handle (renderToFeedbackBuffer fused
+  // into the traversal).  Keys are (stamp << 32) | order-preserving bits of
+  // the error, combined with atomicMax, so a table needs no clearing between
+  // analysis cycles: entries of older stamps are stale.
+  unsigned long long* fb_slot;   // per slot: the largest key of a record whose hit node lies in it
+  unsigned long long* fb_gid;    // per treelet: the largest key of a record asking for it
+  uint32_t* fb_queue;            // treelets asked for in this cycle, first-come order
+  uint32_t* fb_count;            // entries appended to fb_queue (may exceed fb_cap)
+  uint32_t fb_cap;
+  uint32_t fb_stamp;
 };
 
 // One view of a launch: its scene, camera uniforms, outputs and first tile.
@@ -36,7 +46,8 @@ struct TraceFrame {
   svo_frame_out out;
   uint32_t tile_begin;    // first global tile id of this view
   uint32_t tiles_x;       // tiles per row
-  uint32_t pad[2];
+  uint32_t fb_stride;     // record analysis feedback for pixels with x % s == y % s == 0 (0 = off)
+  uint32_t pad;
 };
 
 // Shape of a launch: pixel tile of one wavefront (tile_w x 64/tile_w),
@@ -66,6 +77,11 @@ svo_status stage_views(const svo_view* views, uint32_t n, int policy, const Laun
                        StagedLaunch* out);
 LaunchShape renderer_shape(const svo_renderer* r);
 int renderer_device(const svo_renderer* r);
+
+// Feedback consumption: out[i] = fb_gid[queue[i]] for i < min(*count, cap),
+// then *count = 0 (a kernel, so that it may sit in a captured graph).
+int launch_feedback_gather(const unsigned long long* fb_gid, const uint32_t* queue, uint32_t* count, uint32_t cap,
+                           unsigned long long* out, void* stream);
 
 // Writes the normalised FP32 ray direction of every pixel of `cam`
 // (3 floats per pixel, row-major) -- used to pin the ray generation.

@@ -90,7 +90,16 @@ struct Hit {
   uint32_t node;  // 0xFFFFFFFF = miss
   float t;
   uint32_t pushes;
+  // Analysis record of sampleAnalyse (treeletRenderer_rgba.cl:848-875):
+  // kind 1 = leaf hit, error = scale_exp2 - tScaleRatio*tcMin; kind 2 = inner
+  // node at the LOD / depth limit, error = (tScaleRatio*tcMin > scale_exp2);
+  // kind 0 = no record (miss or maxLoops).
+  uint32_t kind;
+  float err;
 };
+
+__device__ __forceinline__ float leaf_error(float se, float ratio, float tc_min) { return se - (ratio * tc_min); }
+__device__ __forceinline__ float stop_error(float se, float ratio, float tc_min) { return (ratio * tc_min > se) ? 1.0f : 0.0f; }
 
 // tcMax of the node centred at c with half-edge h (the value computed when
 // the node was pushed).
@@ -148,7 +157,7 @@ struct RayStack<SVO_STACK_PRIVATE, BLOCK> {  // private (scratch-backed) arrays
 // chains of a wave.)
 template <int POLICY, typename Stack>
 __device__ Hit traverse(const TracePools& S, const float o[3], float d[3], float ratio, Stack& stk) {
-  Hit r{0xFFFFFFFFu, __int_as_float(0x7F800000), 0u};
+  Hit r{0xFFFFFFFFu, __int_as_float(0x7F800000), 0u, 0u, 0.0f};
   float t_min, t_max;
   if (!root_box(o, d, t_min, t_max)) return r;
   r.t = t_max;
@@ -251,6 +260,8 @@ __device__ Hit traverse(const TracePools& S, const float o[3], float d[3], float
         if ((masks >> ci) & 1u) {  // leaf
           r.node = child;
           r.t = tmin;
+          r.kind = 1u;
+          r.err = leaf_error(se, ratio, tc_min);
           return r;
         }
         bool stop;
@@ -259,6 +270,8 @@ __device__ Hit traverse(const TracePools& S, const float o[3], float d[3], float
         if (stop) {
           r.node = child;
           r.t = tmin;
+          r.kind = 2u;
+          r.err = stop_error(se, ratio, tc_min);
           return r;
         }
         if (child >= S.pool_nodes) {  // corrupt offset: never read outside the pool
@@ -366,6 +379,8 @@ __device__ Hit traverse_fast(const Uniform& U, const float o[3], const float d[3
   // 3 maxLoops, 4 corrupt offset
   uint32_t outcome = 0;
   uint32_t hit_node = 0;
+  uint32_t hit_kind = 0;
+  float hit_err = 0.0f;
 
   do {
     const float se = exp2i(scale - U.smax);  // half-edge of the current node
@@ -424,6 +439,9 @@ __device__ Hit traverse_fast(const Uniform& U, const float o[3], const float d[3
         if (((masks >> ci) & 1u) || stop) {  // leaf, or inner node at the LOD / depth limit
           outcome = 1;
           hit_node = child;
+          const bool is_leaf = (masks >> ci) & 1u;
+          hit_kind = is_leaf ? 1u : 2u;
+          hit_err = is_leaf ? leaf_error(se, ratio, tc_min) : stop_error(se, ratio, tc_min);
         } else if (child >= U.pool_nodes) {
           outcome = 4;
         } else {
@@ -469,6 +487,8 @@ __device__ Hit traverse_fast(const Uniform& U, const float o[3], const float d[3
   if (outcome == 1) {
     r.node = hit_node;
     r.t = tmin;
+    r.kind = hit_kind;
+    r.err = hit_err;
   } else if (outcome == 3) {
     if (POLICY == 0) {  // maxLoops returns the current parent as a hit
       r.node = nidx;
@@ -483,7 +503,7 @@ __device__ Hit traverse_fast(const Uniform& U, const float o[3], const float d[3
 // generic loop.
 template <int POLICY, typename Stack>
 __device__ Hit trace_ray(const TracePools& S, const float o[3], float d[3], float ratio, Stack& stk) {
-  Hit r{0xFFFFFFFFu, __int_as_float(0x7F800000), 0u};
+  Hit r{0xFFFFFFFFu, __int_as_float(0x7F800000), 0u, 0u, 0.0f};
   float t_min, t_max;
   if (!root_box(o, d, t_min, t_max)) return r;
   float dc[3] = {d[0], d[1], d[2]};
@@ -509,6 +529,33 @@ __device__ Hit trace_ray(const TracePools& S, const float o[3], float d[3], floa
     return traverse_fast<POLICY>(U, o, dc, rc, mo, t_min, t_max, ratio, stk, r);
   }
   return traverse<POLICY>(S, o, d, ratio, stk);
+}
+
+// Order-preserving map of a float onto unsigned (larger float -> larger key).
+__device__ __forceinline__ uint32_t float_key(float f) {
+  const uint32_t b = __float_as_uint(f);
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+
+// RenderPassAnalyse::performAnalysePass (src/RenderPassAnalyse.cpp:248-304),
+// first half, on the device: a record marks the treelet holding its node as
+// visible with the largest error seen, and a leaf whose first-child field
+// holds a sub-treelet gid asks for that treelet (largest error wins; the
+// first record of a cycle appends the gid to the queue).
+__device__ __forceinline__ void record_feedback(const TracePools& S, const Hit& hit) {
+  const unsigned long long key = ((unsigned long long)S.fb_stamp << 32) | float_key(hit.err);
+  const uint32_t slot = (S.slot_shift < 32) ? (hit.node >> S.slot_shift) : (hit.node / S.nodes_per_treelet);
+  atomicMax(&S.fb_slot[slot], key);
+  if (hit.kind == 1u) {
+    const uint32_t sub = (uint32_t)S.nodes[hit.node];  // a leaf's first child: its sub-treelet gid, 0 if none
+    if (sub != 0u && sub < S.num_gids) {
+      const unsigned long long old = atomicMax(&S.fb_gid[sub], key);
+      if ((uint32_t)(old >> 32) != S.fb_stamp) {
+        const uint32_t i = atomicAdd(S.fb_count, 1u);
+        if (i < S.fb_cap) S.fb_queue[i] = sub;
+      }
+    }
+  }
 }
 
 template <int POLICY, int TW, int WAVES, int STACK>
@@ -553,11 +600,24 @@ __global__ __launch_bounds__(64 * WAVES) void trace_kernel(const TraceFrame* __r
     tree = (uint32_t)S.slot_gid[slot];
     rgba = S.colors[hit.node] | 0xFFu;
   }
+  if (F.fb_stride != 0u && hit.kind != 0u && hit.node < S.pool_nodes && x % F.fb_stride == 0u &&
+      y % F.fb_stride == 0u)
+    record_feedback(S, hit);
   if (F.out.t) F.out.t[pix] = hit.t;
   if (F.out.node) F.out.node[pix] = hit.node;
   if (F.out.treelet) F.out.treelet[pix] = tree;
   if (F.out.rgba) F.out.rgba[pix] = rgba;
   if (F.out.pushes) F.out.pushes[pix] = hit.pushes;
+}
+
+__global__ void feedback_gather_kernel(const unsigned long long* __restrict__ fb_gid, const uint32_t* __restrict__ queue,
+                                       uint32_t* count, uint32_t cap, unsigned long long* __restrict__ out) {
+  const uint32_t n = min(*count, cap);
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) out[i] = fb_gid[queue[i]];
+}
+
+__global__ void feedback_reset_kernel(uint32_t* count) {
+  if (threadIdx.x == 0) *count = 0u;
 }
 
 __global__ void ray_dir_kernel(svo_camera cam, float* dirs) {
@@ -628,6 +688,15 @@ int launch_trace(const LaunchShape& shape, const TraceFrame* frames_dev, uint32_
   const dim3 grid((num_tiles + shape.waves - 1) / shape.waves);
   const size_t lds = shape.stack == SVO_STACK_LDS ? (size_t)stack_levels * block * sizeof(uint2) : 0;
   hipLaunchKernelGGL(fn, grid, dim3(block), lds, (hipStream_t)stream, frames_dev, num_frames, num_tiles);
+  return (int)hipGetLastError();
+}
+
+int launch_feedback_gather(const unsigned long long* fb_gid, const uint32_t* queue, uint32_t* count, uint32_t cap,
+                           unsigned long long* out, void* stream) {
+  const uint32_t blocks = std::max(1u, std::min(1024u, (cap + 255u) / 256u));
+  hipLaunchKernelGGL(feedback_gather_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream, fb_gid, queue, count, cap,
+                     out);
+  hipLaunchKernelGGL(feedback_reset_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, count);
   return (int)hipGetLastError();
 }
 

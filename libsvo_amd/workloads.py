@@ -16,6 +16,9 @@ half-pixel offset.
 * Batched     : 64 views per launch over the depth-11 terrain and the sphere
                 cloud, resolutions uniform over {256^2, 512^2, 1280x720,
                 1920x1080}.
+* Out-of-core : depth-13 terrain in 16384-node treelets behind an 8 GiB
+                device pool, 300 frames at 1920x1080 along a Catmull-Rom
+                spline through 8 seeded control points 0.1 above the surface.
 * Serving     : Poisson arrivals over Menger depth 9, terrain depth 11,
                 sphere cloud depth 10 and terrain depth 12; resolutions
                 256^2 40 %, 1280x720 40 %, 1920x1080 15 %, 3840x2160 5 %.
@@ -36,6 +39,7 @@ TERRAIN_CAMERA_SEED = 11
 SPHERES_SEED = 5
 BATCH_SEED = 31
 SERVING_SEED = 47
+FLIGHT_SEED = 59
 FAR = 20.0
 FOV = 60.0
 
@@ -127,6 +131,50 @@ def serving_requests(kinds, rate: float, count: int, seed: int = SERVING_SEED, r
         w, h = res[int(picks[i])]
         out.append((k, scene_camera(kinds[k], int(cam_seeds[i]), w, h), float(arrivals[i])))
     return out
+
+
+def flythrough_cameras(frames: int = 300, seed: int = FLIGHT_SEED, width: int = 1920, height: int = 1080,
+                       lod: float = 1.0, terrain_seed: int = TERRAIN_SEED, points: int = 8):
+    """Cameras along a uniform Catmull-Rom spline through `points` seeded
+    control points (x, z uniform in [0.15, 0.85], 0.1 above the surface;
+    end points repeated), never below 0.05 above the surface, looking along
+    the horizontal direction of travel and 15 degrees down."""
+    from . import _capi
+    rng = np.random.default_rng(seed)
+
+    def ground(x, z):
+        h = _capi.ctypes.c_double()
+        _capi.check(_capi.lib.svo_terrain_height(terrain_seed, float(x), float(z), _capi.ctypes.byref(h)),
+                    "svo_terrain_height")
+        return h.value
+
+    ctrl = []
+    for _ in range(points):
+        x, z = rng.uniform(0.15, 0.85, size=2)
+        ctrl.append(np.array([x, ground(x, z) + 0.1, z]))
+    P = [ctrl[0]] + ctrl + [ctrl[-1]]
+
+    def spline(u):
+        seg = min(int(u), points - 2)
+        t = u - seg
+        p0, p1, p2, p3 = P[seg], P[seg + 1], P[seg + 2], P[seg + 3]
+        return 0.5 * ((2 * p1) + (-p0 + p2) * t + (2 * p0 - 5 * p1 + 4 * p2 - p3) * t * t +
+                      (-p0 + 3 * p1 - 3 * p2 + p3) * t * t * t)
+
+    cams = []
+    span = float(points - 1)
+    pitch = math.radians(-15.0)
+    for f in range(frames):
+        u = span * f / max(1, frames - 1)
+        p = spline(u)
+        q = spline(min(span, u + 1e-3)) if u < span else p + (p - spline(u - 1e-3))
+        p[1] = max(p[1], ground(p[0], p[2]) + 0.05)
+        fwd = q - p
+        yaw = math.atan2(fwd[2], fwd[0])
+        look = np.array([math.cos(pitch) * math.cos(yaw), math.sin(pitch), math.cos(pitch) * math.sin(yaw)])
+        eye = p - 0.5
+        cams.append(look_at(eye, eye + look, (0.0, 1.0, 0.0), FOV, width, height, FAR, lod))
+    return cams
 
 
 def incore_bytes_for(scene: Scene, spare_slots: int = 0) -> int:

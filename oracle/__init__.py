@@ -26,7 +26,7 @@ _lib.oracle_free.restype = None
 _lib.oracle_info.argtypes = [vp, ctypes.POINTER(u32), ctypes.POINTER(u32), ctypes.POINTER(u32)]
 _lib.oracle_make_incore.argtypes = [vp, u64]
 _lib.oracle_incore.argtypes = [vp] + [ctypes.POINTER(vp)] * 4 + [ctypes.POINTER(u64), ctypes.POINTER(u32)]
-_lib.oracle_render_buffers.argtypes = [vp, u64, vp, vp, u32, u32, u32, vp, ctypes.c_int] + [vp] * 9 + [ctypes.c_int]
+_lib.oracle_render_buffers.argtypes = [vp, u64, vp, vp, u32, u32, u32, vp, ctypes.c_int] + [vp] * 12 + [ctypes.c_int]
 _lib.oracle_render.argtypes = [vp, vp, ctypes.c_int] + [vp] * 9 + [ctypes.c_int]
 _lib.oracle_trace_ray.argtypes = [vp, vp, vp, ctypes.c_float, ctypes.c_int, vp, vp, vp, vp, vp]
 _lib.oracle_max_threads.restype = ctypes.c_int
@@ -121,8 +121,9 @@ class OracleScene:
 
 
 def render_buffers(nodes: np.ndarray, colors: np.ndarray, slot_gid: np.ndarray, nodes_per_treelet: int, depth: int,
-                   cam, policy="treelet", threads: int = 0) -> Dict[str, np.ndarray]:
-    """Oracle traversal over caller-provided incore arrays."""
+                   cam, policy="treelet", threads: int = 0, records: bool = False) -> Dict[str, np.ndarray]:
+    """Oracle traversal over caller-provided incore arrays.  records=True adds
+    the per-ray analysis record (rec_kind, rec_err, rec_sub)."""
     nodes = np.ascontiguousarray(nodes, np.uint32).reshape(-1, 2)
     colors = np.ascontiguousarray(colors, np.uint32)
     slot_gid = np.ascontiguousarray(slot_gid, np.int32)
@@ -130,10 +131,15 @@ def render_buffers(nodes: np.ndarray, colors: np.ndarray, slot_gid: np.ndarray, 
     out = {"t": np.empty((H, W), np.float32), "node": np.empty((H, W), np.uint32),
            "treelet": np.empty((H, W), np.uint32), "rgba": np.empty((H, W), np.uint32),
            "boundary": np.empty((H, W), np.uint8), "pushes": np.empty((H, W), np.uint32)}
+    if records:
+        out.update({"rec_kind": np.empty((H, W), np.uint8), "rec_err": np.empty((H, W), np.float32),
+                    "rec_sub": np.empty((H, W), np.uint32)})
+    g = out.get
     r = _lib.oracle_render_buffers(_ptr(nodes), nodes.shape[0], _ptr(colors), _ptr(slot_gid), nodes_per_treelet,
                                    slot_gid.size, depth, ctypes.byref(cam), POLICY.get(policy, policy),
                                    _ptr(out["t"]), _ptr(out["node"]), _ptr(out["treelet"]), _ptr(out["rgba"]),
                                    _ptr(out["boundary"]), _ptr(out["pushes"]), None, None, None,
+                                   _ptr(g("rec_kind")), _ptr(g("rec_err")), _ptr(g("rec_sub")),
                                    threads if threads > 0 else max_threads())
     if r != 0:
         raise RuntimeError(f"oracle_render_buffers failed with {r}")

@@ -194,7 +194,15 @@ struct Result {
   float center[3];
   uint32_t pushes;
   bool boundary;
+  // analysis record (sampleAnalyse, opencl/treeletRenderer_rgba.cl:848-875):
+  // kind 1 leaf hit, 2 inner node at the LOD / depth limit, 0 none
+  uint32_t rec_kind;
+  float rec_err;
+  uint32_t rec_sub;  // leaf's first-child field (sub-treelet gid), 0 otherwise
 };
+
+// A leaf record's sub-treelet gid is the leaf node's own first-child field.
+inline void leaf_record(const Buffers& B, uint32_t leaf, float scale_exp2, float ratio, float tc_min, Result* r);
 
 inline float cl_max(float x, float y) { return (x < y) ? y : x; }
 inline float cl_min(float x, float y) { return (y < x) ? y : x; }
@@ -283,7 +291,21 @@ struct StackElement {
 constexpr int kMaxStack = 24;
 
 // Policy A: treeletRenderer_rgba.cl::sample, scaleMax = max(13, D+1).
+inline void leaf_record(const Buffers& B, uint32_t leaf, float scale_exp2, float ratio, float tc_min, Result* r) {
+  r->rec_kind = 1;
+  r->rec_err = scale_exp2 - (ratio * tc_min);
+  r->rec_sub = leaf < B.pool ? B.nodes[leaf].first : 0u;
+}
+inline void stop_record(float scale_exp2, float ratio, float tc_min, Result* r) {
+  r->rec_kind = 2;
+  r->rec_err = (ratio * tc_min > scale_exp2) ? 1.0f : 0.0f;
+  r->rec_sub = 0;
+}
+
 void sample_treelet(const Buffers& B, const float o[3], float d[3], float ratio, Result* r) {
+  r->rec_kind = 0;
+  r->rec_err = 0.0f;
+  r->rec_sub = 0;
   r->hit = false;
   r->loops = 0;
   r->pushes = 0;
@@ -364,9 +386,11 @@ void sample_treelet(const Buffers& B, const float o[3], float d[3], float ratio,
           r->t = parent->parent_tmin;
           r->loops = while_counter;
           std::memcpy(r->center, cc, sizeof cc);
+          leaf_record(B, child, scale_exp2, ratio, tc_min, r);
           return;
         }
         if (ratio * tc_min > scale_exp2 || scale_max - scale == max_depth_cap) {
+          stop_record(scale_exp2, ratio, tc_min, r);
           r->hit = true;
           r->node = child;
           r->depth = scale_max - scale + 1;
@@ -396,6 +420,9 @@ void sample_treelet(const Buffers& B, const float o[3], float d[3], float ratio,
 
 // Policy C: CpuRaycasterSingleRay3::traversSvo, scaleMax = SVO depth.
 void travers_cpu3(const Buffers& B, const float o[3], float d[3], float ratio, Result* r) {
+  r->rec_kind = 0;
+  r->rec_err = 0.0f;
+  r->rec_sub = 0;
   r->hit = false;
   r->loops = 0;
   r->pushes = 0;
@@ -473,6 +500,8 @@ void travers_cpu3(const Buffers& B, const float o[3], float d[3], float ratio, R
       if (valid_flag(pn.masks, child_index)) {
         const uint32_t child = parent_node_index + nth_child_idx(pn.masks, pn.first, child_index);
         if (leaf_flag(pn.masks, child_index) || ratio * tc_max > scale_exp2 || scale == 0) {
+          if (leaf_flag(pn.masks, child_index)) leaf_record(B, child, scale_exp2, ratio, tc_min, r);
+          else stop_record(scale_exp2, ratio, tc_min, r);
           r->hit = true;
           r->node = child;
           r->depth = scale_max - scale;
@@ -546,10 +575,14 @@ int oracle_incore(const oracle_scene* s, const uint32_t** nodes, const uint32_t*
 //   pushes                 : number of stack pushes of the ray
 //   depth, loops           : SampleResult depth / numWhileLoops
 //   center                 : 3 floats, nodeCenter of the hit
+//   rec_kind, rec_err,     : analysis record of the ray (sampleAnalyse):
+//   rec_sub                  kind 1 leaf / 2 LOD-depth stop / 0 none, its
+//                            error, the leaf's sub-treelet gid
 int oracle_render_buffers(const uint32_t* nodes, uint64_t pool_nodes, const uint32_t* colors, const int32_t* slot_gid,
                           uint32_t nodes_per_treelet, uint32_t num_slots, uint32_t depth, const void* camera,
                           int policy, float* t, uint32_t* node, uint32_t* treelet, uint32_t* rgba, uint8_t* boundary,
-                          uint32_t* pushes, uint32_t* depth_out, uint32_t* loops, float* center, int threads) {
+                          uint32_t* pushes, uint32_t* depth_out, uint32_t* loops, float* center, uint8_t* rec_kind,
+                          float* rec_err, uint32_t* rec_sub, int threads) {
   if (!nodes || !colors || !slot_gid || !camera || nodes_per_treelet == 0) return 1;
   if (policy != 0 && policy != 2) return 1;
   Camera cam;
@@ -581,6 +614,9 @@ int oracle_render_buffers(const uint32_t* nodes, uint64_t pool_nodes, const uint
       if (depth_out) depth_out[p] = r.depth;
       if (loops) loops[p] = r.loops;
       if (center) { center[3 * p] = r.center[0]; center[3 * p + 1] = r.center[1]; center[3 * p + 2] = r.center[2]; }
+      if (rec_kind) rec_kind[p] = (uint8_t)r.rec_kind;
+      if (rec_err) rec_err[p] = r.rec_err;
+      if (rec_sub) rec_sub[p] = r.rec_sub;
     }
   }
   return 0;
@@ -592,7 +628,7 @@ int oracle_render(const oracle_scene* s, const void* camera, int policy, float* 
   if (!s || s->incore.empty()) return 6;
   return oracle_render_buffers(reinterpret_cast<const uint32_t*>(s->incore.data()), s->incore.size(), s->colors.data(),
                                s->slot_gid.data(), s->N, s->num_slots, s->max_depth, camera, policy, t, node, treelet,
-                               rgba, boundary, pushes, depth_out, loops, center, threads);
+                               rgba, boundary, pushes, depth_out, loops, center, nullptr, nullptr, nullptr, threads);
 }
 
 // One ray with explicit origin / direction (direction used as given).
