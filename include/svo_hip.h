@@ -129,6 +129,26 @@ svo_status svo_scene_get_treelet(const svo_scene* scene, uint32_t gid,
  * inner attributes = mean of the children. */
 enum { SVO_BUILD_PACK_BFS = 1u };
 
+/* Legacy single-SVO files (one octree, absolute first-child indices), read
+ * into a scene of one treelet (relative offsets, gid 0), which every
+ * traversal policy renders; node indices are unchanged:
+ *   SVO_SINGLE_SVOLOADER  SvoLoader (src/SvoLoader.cpp:80-107): ASCII
+ *                         "<depth> <numNodes> ", then the nodes; attribute
+ *                         package i belongs to node i.
+ *   SVO_SINGLE_V1         SvoBranch v1 (src/incore/SvoBranch.cpp:320-395):
+ *                         ASCII "v1 <depth> <numNodes> ", the nodes, then
+ *                         numNodes u32 attribute indices (the indirection of
+ *                         fillFrameBuffer_rgba.cl:640, resolved at load).
+ * <base>.ia holds one block of packages in this library's .ia layout.
+ * svo_scene_open_single detects the format from the first word. */
+enum { SVO_SINGLE_SVOLOADER = 1u, SVO_SINGLE_V1 = 2u };
+svo_status svo_scene_open_single(const char* base_path, svo_scene** out);
+/* Writes a one-treelet scene as a single-SVO file; attrib_index (v1 only,
+ * may be NULL = identity) is a permutation: node i's package is written at
+ * attrib_index[i]. */
+svo_status svo_scene_write_single(const svo_scene* scene, const char* base_path,
+                                  uint32_t format, const uint32_t* attrib_index);
+
 /* Builds a treelet scene from leaf voxels at depth `depth`: `morton` holds
  * strictly increasing Morton codes (3 bits per level, x most significant,
  * i.e. child index 4x+2y+z), `attribs` two words per leaf
@@ -265,11 +285,16 @@ svo_status svo_mm_read_device_incore(const svo_mm* mm, uint64_t first,
 /* ------------------------------------------------------------ rendering */
 
 /* Traversal policy.  SVO_POLICY_TREELET is the OpenCL renderer's
- * sample() (opencl/treeletRenderer_rgba.cl:201-382); SVO_POLICY_CPU3 is
+ * sample() (opencl/treeletRenderer_rgba.cl:201-382); SVO_POLICY_SINGLE is
+ * the single-SVO OpenCL renderer's sample()
+ * (opencl/fillFrameBuffer_rgba.cl:239-425: scaleMax 15, epsilon 2^-16, LOD
+ * on tcMax, maxLoops = miss; its absolute child indexing addresses the same
+ * nodes as relative indexing on a scene held as one treelet, which is how
+ * svo_scene_open_single stores a single SVO); SVO_POLICY_CPU3 is
  * CpuRaycasterSingleRay3::traversSvo
- * (src/incore/CpuRaycasterSingleRay3.cpp:158-387) run on the same
+ * (src/incore/CpuRaycasterSingleRay3.cpp:158-387).  Every policy runs on the
  * relative-indexed incore buffer. */
-enum { SVO_POLICY_TREELET = 0, SVO_POLICY_CPU3 = 2 };
+enum { SVO_POLICY_TREELET = 0, SVO_POLICY_SINGLE = 1, SVO_POLICY_CPU3 = 2 };
 
 /* Per-camera outputs, device pointers, width*height elements each, row y
  * at offset y*width (y as in renderToBuffer's get_global_id(1)).  Any

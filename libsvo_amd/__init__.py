@@ -31,7 +31,7 @@ __all__ = [
     "device_count", "look_at", "SvoError", "POLICIES", "MISS", "BACKGROUND_RGBA",
 ]
 
-POLICIES = {"treelet": SVO_POLICY_TREELET, "cpu3": SVO_POLICY_CPU3}
+POLICIES = {"treelet": SVO_POLICY_TREELET, "single": _capi.SVO_POLICY_SINGLE, "cpu3": SVO_POLICY_CPU3}
 Camera = svo_camera
 
 
@@ -90,6 +90,20 @@ class Scene:
     @classmethod
     def open(cls, base_path: str) -> "Scene":
         return cls._build(lib.svo_scene_open, base_path.encode())
+
+    @classmethod
+    def open_single(cls, base_path: str) -> "Scene":
+        """A legacy single-SVO file (SvoLoader or SvoBranch v1) as a scene of
+        one treelet."""
+        return cls._build(lib.svo_scene_open_single, base_path.encode())
+
+    def write_single(self, base_path: str, fmt: str = "v1", attrib_index=None) -> None:
+        """Writes a one-treelet scene as a single-SVO file ("v1" or
+        "svoloader"); attrib_index permutes the attribute packages (v1)."""
+        code = {"svoloader": _capi.SVO_SINGLE_SVOLOADER, "v1": _capi.SVO_SINGLE_V1}[fmt]
+        idx = None if attrib_index is None else np.ascontiguousarray(attrib_index, np.uint32)
+        check(lib.svo_scene_write_single(self._h, base_path.encode(), code,
+                                         None if idx is None else idx.ctypes.data), "svo_scene_write_single")
 
     @classmethod
     def menger(cls, level: int, depth: int, seed: int, treelet_bytes: int = 0, packing: str = "dense") -> "Scene":

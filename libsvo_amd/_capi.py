@@ -19,8 +19,10 @@ P = ctypes.POINTER
 SVO_OK = 0
 SVO_ERR_ARG, SVO_ERR_IO, SVO_ERR_FORMAT, SVO_ERR_HIP, SVO_ERR_NOMEM, SVO_ERR_STATE, SVO_ERR_CAPACITY = range(1, 8)
 SVO_POLICY_TREELET = 0
+SVO_POLICY_SINGLE = 1
 SVO_POLICY_CPU3 = 2
 SVO_BUILD_PACK_BFS = 1
+SVO_SINGLE_SVOLOADER, SVO_SINGLE_V1 = 1, 2
 SVO_STACK_LDS, SVO_STACK_PRIVATE = 0, 1
 MISS = 0xFFFFFFFF
 BACKGROUND_RGBA = 0x334D33FF
@@ -105,6 +107,8 @@ SIGNATURES = {
     "svo_scene_get_info": (ctypes.c_int, [vp, P(svo_scene_info)]),
     "svo_scene_get_treelet": (ctypes.c_int, [vp, u32, P(svo_treelet_header), P(vp), P(vp)]),
     "svo_scene_build_from_leaves": (ctypes.c_int, [u32, vp, vp, ctypes.c_size_t, u32, u32, P(vp)]),
+    "svo_scene_open_single": (ctypes.c_int, [ctypes.c_char_p, P(vp)]),
+    "svo_scene_write_single": (ctypes.c_int, [vp, ctypes.c_char_p, u32, vp]),
     "svo_scene_build_menger": (ctypes.c_int, [u32, u32, u64, u32, u32, P(vp)]),
     "svo_scene_build_terrain": (ctypes.c_int, [u32, u64, u32, u32, P(vp)]),
     "svo_scene_build_spheres": (ctypes.c_int, [u32, u32, f32, f32, u64, u32, u32, P(vp)]),

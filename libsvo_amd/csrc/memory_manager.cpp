@@ -14,6 +14,8 @@
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <cstdlib>
+#include <type_traits>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -45,6 +47,43 @@ struct SlotInfo {  // TreeletMemoryManager::SlotInfo (src/TreeletMemoryManager.h
     }                                                                                   \
   } while (0)
 
+// Host incore arrays are page-locked (hipHostMalloc) when the handle has a
+// device, so that uploads are DMA transfers that need no staging copy;
+// host-only handles use ordinary memory.
+template <typename T>
+struct HostAlloc {
+  using value_type = T;
+  using propagate_on_container_move_assignment = std::true_type;
+  using propagate_on_container_copy_assignment = std::true_type;
+  using propagate_on_container_swap = std::true_type;
+  using is_always_equal = std::false_type;
+  bool pinned = false;
+  HostAlloc() = default;
+  explicit HostAlloc(bool p) : pinned(p) {}
+  template <typename U>
+  HostAlloc(const HostAlloc<U>& o) : pinned(o.pinned) {}
+  T* allocate(size_t n) {
+    void* p = nullptr;
+    if (pinned) {
+      if (hipHostMalloc(&p, n * sizeof(T), hipHostMallocDefault) != hipSuccess) throw std::bad_alloc();
+    } else {
+      p = std::malloc(n * sizeof(T));
+      if (!p) throw std::bad_alloc();
+    }
+    return static_cast<T*>(p);
+  }
+  void deallocate(T* p, size_t) {
+    if (pinned) hipHostFree(p);
+    else std::free(p);
+  }
+  template <typename U>
+  bool operator==(const HostAlloc<U>& o) const { return pinned == o.pinned; }
+  template <typename U>
+  bool operator!=(const HostAlloc<U>& o) const { return pinned != o.pinned; }
+};
+template <typename T>
+using HostVec = std::vector<T, HostAlloc<T>>;
+
 }  // namespace
 
 struct svo_mm {
@@ -53,8 +92,8 @@ struct svo_mm {
   uint32_t num_slots = 0;
   int device = 0;
   // host incore buffer (SoA attributes)
-  std::vector<svo_node> incore;
-  std::vector<uint32_t> normals, colors;
+  HostVec<svo_node> incore;
+  HostVec<uint32_t> normals, colors;
   std::vector<SlotInfo> slots;
   std::vector<int32_t> treelet_slot;                 // Treelet::getSlotGid, -1 = not resident
   std::vector<std::set<uint32_t>> children_resident;  // _childTreeletsInIncoreBuffer
@@ -364,6 +403,11 @@ extern "C" svo_status svo_mm_create(const svo_scene* scene, uint64_t incore_byte
   mm->num_slots = (uint32_t)slots;
   mm->device = device;
   try {
+    if (device >= 0) HIP_TRY(hipSetDevice(device));
+    const bool pinned = device >= 0;
+    mm->incore = HostVec<svo_node>(HostAlloc<svo_node>(pinned));
+    mm->normals = HostVec<uint32_t>(HostAlloc<uint32_t>(pinned));
+    mm->colors = HostVec<uint32_t>(HostAlloc<uint32_t>(pinned));
     mm->incore.resize(pool_nodes);
     mm->normals.resize(pool_nodes);
     mm->colors.resize(pool_nodes);
@@ -549,6 +593,16 @@ svo_status mm_trace_pools(const svo_mm* mm, int policy, TracePools* P) {
     P->scale_max = std::max(1u, D);
     P->lod_cap = 0;
     P->max_loops = (D + 1) * (D + 1) * 5;
+  } else if (policy == SVO_POLICY_SINGLE) {
+    // scaleMax 15, maxLoops (15+1)^2 * 5 (fillFrameBuffer_rgba.cl:256,290)
+    if (D > 15) { set_error("the single-SVO policy reaches depth 15 at most (scene depth %u)", D); return SVO_ERR_ARG; }
+    if (mm->scene->treelets.size() != 1 || mm->treelet_slot[0] != 0) {
+      set_error("the single-SVO policy renders scenes of one treelet (in slot 0)");
+      return SVO_ERR_ARG;
+    }
+    P->scale_max = 15;
+    P->lod_cap = 0;
+    P->max_loops = 16 * 16 * 5;
   } else {
     set_error("unknown traversal policy %d", policy);
     return SVO_ERR_ARG;

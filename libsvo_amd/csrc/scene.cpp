@@ -376,6 +376,8 @@ using FilePtr = std::unique_ptr<FILE, FileCloser>;
 bool write_u32(FILE* f, uint32_t v) { return fwrite(&v, 4, 1, f) == 1; }
 bool read_u32(FILE* f, uint32_t* v) { return fread(v, 4, 1, f) == 1; }
 
+}  // namespace
+
 // Depth of every node of every treelet, checking that child offsets stay
 // inside the treelet.  Returns the deepest leaf depth, or -1 on a bad offset.
 int scene_depth(const svo_scene& sc, std::string* why) {
@@ -424,8 +426,6 @@ int scene_depth(const svo_scene& sc, std::string* why) {
   }
   return maxd;
 }
-
-}  // namespace
 
 }  // namespace svo_impl
 

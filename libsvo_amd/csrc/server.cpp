@@ -280,7 +280,7 @@ extern "C" svo_status svo_server_create(svo_renderer* r, const svo_server_config
     set_error("max_batch_views must be 1..%u", kMaxBatchViews);
     return SVO_ERR_ARG;
   }
-  if (cfg->policy != SVO_POLICY_TREELET && cfg->policy != SVO_POLICY_CPU3) {
+  if (cfg->policy != SVO_POLICY_TREELET && cfg->policy != SVO_POLICY_CPU3 && cfg->policy != SVO_POLICY_SINGLE) {
     set_error("unknown traversal policy %d", cfg->policy);
     return SVO_ERR_ARG;
   }

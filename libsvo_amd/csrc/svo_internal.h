@@ -79,6 +79,10 @@ struct svo_scene {
 
 namespace svo_impl {
 
+// Deepest leaf depth of a scene (root = 0), checking that every child offset
+// stays inside its treelet; -1 (and *why) on a bad offset.  scene.cpp.
+int scene_depth(const svo_scene& sc, std::string* why);
+
 struct TracePools;
 
 // Residency handles as seen by the renderer (memory_manager.cpp).

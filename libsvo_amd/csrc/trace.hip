@@ -51,6 +51,40 @@ __device__ __forceinline__ float cl_sign(float x) {
 // 2^k for k in [-126, 127], exact.
 __device__ __forceinline__ float exp2i(int k) { return __int_as_float((127 + k) << 23); }
 
+// Traversal policies (the template parameter POLICY):
+//   0  treeletRenderer_rgba.cl::sample (:201-382): epsilon 1e-5, |d| < 1e-4
+//      clamped to 1e-4*sign(d), LOD stop on tcMin or the depth cap, maxLoops
+//      returns the parent as a hit;
+//   1  fillFrameBuffer_rgba.cl::sample (:239-425), the single-SVO renderer:
+//      epsilon 2^-16, |d| < epsilon clamped to epsilon*sign(d)*10, LOD stop
+//      on tcMax, maxLoops is a miss, absolute child indexing (on a scene
+//      held as one treelet in slot 0 this is the same node as relative
+//      indexing, which is how the buffer is stored);
+//   2  CpuRaycasterSingleRay3::traversSvo (src/incore/CpuRaycasterSingleRay3.
+//      cpp:158-387): epsilon 1e-5, |d| < 1e-5 clamped to 1e-5*sign(d)*10, an
+//      extra pop unless tmin < tmax, LOD stop on tcMax or scale 0, maxLoops
+//      is a miss.
+template <int POLICY>
+__device__ __forceinline__ float policy_eps() { return POLICY == 1 ? 1.52587890625e-05f : 0.00001f; }
+template <int POLICY>
+__device__ __forceinline__ void clamp_direction(float d[3]) {
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    if (POLICY == 0) {
+      if (fabsf(d[a]) < 0.0001f) d[a] = 0.0001f * cl_sign(d[a]);
+    } else {
+      const float e = policy_eps<POLICY>();
+      if (fabsf(d[a]) < e) d[a] = e * cl_sign(d[a]) * 10.0f;
+    }
+  }
+}
+template <int POLICY>
+__device__ __forceinline__ bool policy_pop(float tmin, float tmax) {
+  const float eps = policy_eps<POLICY>();
+  if (POLICY == 2) return (tmax < 0.0f) || (fabsf(tmin - tmax) < eps) || !(tmin < tmax);
+  return (tmax < 0.0f) || !(fabsf(tmin - tmax) > eps);
+}
+
 // intersectAABB against [-0.5,0.5]^3 (opencl/treeletRenderer_rgba.cl:100-178).
 __device__ __forceinline__ bool root_box(const float o[3], const float d[3], float& t_min, float& t_max) {
   float t_near = -10000000.0f, t_far = 10000000.0f;
@@ -162,15 +196,8 @@ __device__ Hit traverse(const TracePools& S, const float o[3], float d[3], float
   if (!root_box(o, d, t_min, t_max)) return r;
   r.t = t_max;
 
-  const float eps = 0.00001f;
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    if (POLICY == 0) {
-      if (fabsf(d[a]) < 0.0001f) d[a] = 0.0001f * cl_sign(d[a]);
-    } else {
-      if (fabsf(d[a]) < eps) d[a] = eps * cl_sign(d[a]) * 10.0f;
-    }
-  }
+  const float eps = policy_eps<POLICY>();
+  clamp_direction<POLICY>(d);
   float rc[3], mo[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
@@ -226,9 +253,7 @@ __device__ Hit traverse(const TracePools& S, const float o[3], float d[3], float
       }
       return r;
     }
-    bool pop;
-    if (POLICY == 0) pop = (tmax < 0.0f) || !(fabsf(tmin - tmax) > eps);
-    else pop = (tmax < 0.0f) || (fabsf(tmin - tmax) < eps) || !(tmin < tmax);
+    const bool pop = policy_pop<POLICY>(tmin, tmax);
 
     bool fetch = false;
     uint32_t fidx = 0;
@@ -256,7 +281,7 @@ __device__ Hit traverse(const TracePools& S, const float o[3], float d[3], float
 
       const uint32_t masks = (uint32_t)(nd >> 32);
       if ((masks >> (8u + ci)) & 1u) {
-        const uint32_t child = nidx + (uint32_t)nd + (uint32_t)__popc((masks >> 8) & ((1u << ci) - 1u));
+        const uint32_t child = nidx + (uint32_t)nd + (uint32_t)__popc((masks >> 8) & ((1u << ci) - 1u));  // relative
         if ((masks >> ci) & 1u) {  // leaf
           r.node = child;
           r.t = tmin;
@@ -266,6 +291,7 @@ __device__ Hit traverse(const TracePools& S, const float o[3], float d[3], float
         }
         bool stop;
         if (POLICY == 0) stop = (ratio * tc_min > se) || (S.lod_cap != 0 && (uint32_t)(smax - scale) == S.lod_cap);
+        else if (POLICY == 1) stop = ratio * tc_max > se;
         else stop = (ratio * tc_max > se) || scale == 0;
         if (stop) {
           r.node = child;
@@ -359,7 +385,7 @@ __device__ __forceinline__ Uniform make_uniform(const TracePools& S) {
 template <int POLICY, typename Stack>
 __device__ Hit traverse_fast(const Uniform& U, const float o[3], const float d[3], const float rc[3], const float mo[3],
                              float t_min, float t_max, float ratio, Stack& stk, Hit r) {
-  const float eps = 0.00001f;
+  const float eps = policy_eps<POLICY>();
   bool neg[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) neg[a] = rc[a] < 0.0f;
@@ -406,9 +432,7 @@ __device__ Hit traverse_fast(const Uniform& U, const float o[3], const float d[3
       outcome = 3;
       break;
     }
-    bool pop;
-    if (POLICY == 0) pop = (tmax < 0.0f) || !(fabsf(tmin - tmax) > eps);
-    else pop = (tmax < 0.0f) || (fabsf(tmin - tmax) < eps) || !(tmin < tmax);
+    const bool pop = policy_pop<POLICY>(tmin, tmax);
 
     bool fetch = false;
     uint32_t fidx = 0;
@@ -432,9 +456,10 @@ __device__ Hit traverse_fast(const Uniform& U, const float o[3], const float d[3
       const float tc_max = cl_min(t1[0], cl_min(t1[1], t1[2]));
       const uint32_t masks = (uint32_t)(nd >> 32);
       if ((masks >> (8u + ci)) & 1u) {
-        const uint32_t child = nidx + (uint32_t)nd + (uint32_t)__popc((masks >> 8) & ((1u << ci) - 1u));
+        const uint32_t child = nidx + (uint32_t)nd + (uint32_t)__popc((masks >> 8) & ((1u << ci) - 1u));  // relative
         bool stop;
         if (POLICY == 0) stop = (ratio * tc_min > se) || scale == U.lod_scale;
+        else if (POLICY == 1) stop = ratio * tc_max > se;
         else stop = (ratio * tc_max > se) || scale == 0;
         if (((masks >> ci) & 1u) || stop) {  // leaf, or inner node at the LOD / depth limit
           outcome = 1;
@@ -507,14 +532,7 @@ __device__ Hit trace_ray(const TracePools& S, const float o[3], float d[3], floa
   float t_min, t_max;
   if (!root_box(o, d, t_min, t_max)) return r;
   float dc[3] = {d[0], d[1], d[2]};
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    if (POLICY == 0) {
-      if (fabsf(dc[a]) < 0.0001f) dc[a] = 0.0001f * cl_sign(dc[a]);
-    } else {
-      if (fabsf(dc[a]) < 0.00001f) dc[a] = 0.00001f * cl_sign(dc[a]) * 10.0f;
-    }
-  }
+  clamp_direction<POLICY>(dc);
   float rc[3], mo[3];
   bool finite = true;
 #pragma unroll
@@ -683,7 +701,9 @@ int launch_trace(const LaunchShape& shape, const TraceFrame* frames_dev, uint32_
                  uint32_t stack_levels, int policy, void* stream) {
   if (num_tiles == 0) return 0;
   if (!shape_supported(shape) || stack_levels > (uint32_t)kMaxScale) return (int)hipErrorInvalidValue;
-  const TraceFn fn = policy == SVO_POLICY_CPU3 ? pick_tile<2>(shape) : pick_tile<0>(shape);
+  const TraceFn fn = policy == SVO_POLICY_CPU3     ? pick_tile<2>(shape)
+                     : policy == SVO_POLICY_SINGLE ? pick_tile<1>(shape)
+                                                   : pick_tile<0>(shape);
   const uint32_t block = 64u * shape.waves;
   const dim3 grid((num_tiles + shape.waves - 1) / shape.waves);
   const size_t lds = shape.stack == SVO_STACK_LDS ? (size_t)stack_levels * block * sizeof(uint2) : 0;

@@ -32,7 +32,12 @@ _lib.oracle_trace_ray.argtypes = [vp, vp, vp, ctypes.c_float, ctypes.c_int, vp, 
 _lib.oracle_max_threads.restype = ctypes.c_int
 _lib.oracle_ray_dirs.argtypes = [vp, vp]
 
-POLICY = {"treelet": 0, "cpu3": 2}
+_lib.oracle_open_single.argtypes = [ctypes.c_char_p, ctypes.POINTER(vp)]
+_lib.oracle_free_single.argtypes = [vp]
+_lib.oracle_free_single.restype = None
+_lib.oracle_render_single.argtypes = [vp, vp] + [vp] * 6 + [ctypes.c_int]
+
+POLICY = {"treelet": 0, "single": 1, "cpu3": 2}
 
 
 def max_threads() -> int:
@@ -118,6 +123,36 @@ class OracleScene:
         if r != 0:
             raise RuntimeError("oracle_trace_ray failed")
         return {"node": node.value, "t": t.value, "depth": depth.value, "center": tuple(c), "boundary": bool(b.value)}
+
+
+class OracleSingle:
+    """A legacy single-SVO file (SvoLoader or SvoBranch v1) read by the
+    oracle's own parser and traversed with absolute indexing (policy B,
+    fillFrameBuffer_rgba.cl::sample)."""
+
+    def __init__(self, base_path: str):
+        self._h = vp()
+        r = _lib.oracle_open_single(base_path.encode(), ctypes.byref(self._h))
+        if r != 0:
+            raise RuntimeError(f"oracle_open_single({base_path}) failed with {r}")
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _lib.oracle_free_single(self._h)
+            self._h = None
+
+    def render(self, cam, threads: int = 0) -> Dict[str, np.ndarray]:
+        W, H = cam.width, cam.height
+        out = {"t": np.empty((H, W), np.float32), "node": np.empty((H, W), np.uint32),
+               "treelet": np.empty((H, W), np.uint32), "rgba": np.empty((H, W), np.uint32),
+               "boundary": np.empty((H, W), np.uint8), "pushes": np.empty((H, W), np.uint32)}
+        r = _lib.oracle_render_single(self._h, ctypes.byref(cam), *[_ptr(out[k]) for k in
+                                                                    ("t", "node", "treelet", "rgba", "boundary",
+                                                                     "pushes")],
+                                      threads if threads > 0 else max_threads())
+        if r != 0:
+            raise RuntimeError("oracle_render_single failed")
+        return out
 
 
 def render_buffers(nodes: np.ndarray, colors: np.ndarray, slot_gid: np.ndarray, nodes_per_treelet: int, depth: int,

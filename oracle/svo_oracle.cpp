@@ -527,6 +527,146 @@ void travers_cpu3(const Buffers& B, const float o[3], float d[3], float ratio, R
   }
 }
 
+// Policy B: fillFrameBuffer_rgba.cl::sample (:239-425), the single-SVO
+// renderer, over a node array with ABSOLUTE first-child indices:
+// scaleMax 15, epsilon pow(2, -16), |d| < epsilon -> epsilon*sign(d)*10,
+// getNthChildIndex = first + number of valid children before idx, LOD stop
+// on tcMax, maxLoops (16^2*5) ends the loop as a miss.
+void sample_fill(const Node* nodes, uint64_t n, const float o[3], float d[3], float ratio, Result* r) {
+  r->rec_kind = 0;
+  r->rec_err = 0.0f;
+  r->rec_sub = 0;
+  r->hit = false;
+  r->loops = 0;
+  r->pushes = 0;
+  r->boundary = false;
+  r->node = 0xFFFFFFFFu;
+  r->depth = 0;
+  r->t = INFINITY;
+  r->center[0] = r->center[1] = r->center[2] = 0.0f;
+  float t_min, t_max;
+  if (!intersect_aabb(o, d, &t_min, &t_max)) return;
+  const int scale_max = 15;
+  int scale = scale_max - 1;
+  float scale_exp2 = 0.5f;
+  const float epsilon = std::ldexp(1.0f, -scale_max - 1);
+  StackElement stack[kMaxStack];
+  stack[scale] = StackElement{0, t_min, t_max, {0.0f, 0.0f, 0.0f}};
+  if (small_component(d)) r->boundary = true;
+  for (int a = 0; a < 3; ++a)
+    if (std::fabs(d[a]) < epsilon) d[a] = epsilon * cl_sign(d[a]) * 10.0f;
+  const float dx_r = 1.0f / d[0], mpx = -o[0] * dx_r;
+  const float dy_r = 1.0f / d[1], mpy = -o[1] * dy_r;
+  const float dz_r = 1.0f / d[2], mpz = -o[2] * dz_r;
+  unsigned while_counter = 0;
+  const unsigned max_loops = (unsigned)((scale_max + 1) * (scale_max + 1) * 5.0f);
+  while (scale < scale_max) {
+    ++while_counter;
+    if (while_counter == max_loops) { r->boundary = true; break; }
+    StackElement* parent = &stack[scale];
+    scale_exp2 = std::ldexp(1.0f, scale - scale_max);
+    const float child_size_half = scale_exp2 * 0.5f;
+    if (near_pop_threshold(parent->parent_tmin, parent->parent_tmax, epsilon)) r->boundary = true;
+    if (parent->parent_tmax < 0.0f) { ++scale; continue; }
+    if (std::fabs(parent->parent_tmin - parent->parent_tmax) > epsilon) {
+      float e[3];
+      for (int a = 0; a < 3; ++a) e[a] = (o[a] + (parent->parent_tmin + epsilon) * d[a]) - parent->center[a];
+      const int idx = 4 * (e[0] > 0.0f) + 2 * (e[1] > 0.0f) + (e[2] > 0.0f);
+      float cc[3];
+      cc[0] = (-0.5f + (bool)(idx & 4)) * scale_exp2 + parent->center[0];
+      cc[1] = (-0.5f + (bool)(idx & 2)) * scale_exp2 + parent->center[1];
+      cc[2] = (-0.5f + (bool)(idx & 1)) * scale_exp2 + parent->center[2];
+      float tx0 = dx_r * (cc[0] - child_size_half) + mpx, tx1 = dx_r * (cc[0] + child_size_half) + mpx;
+      swap_to_increasing(tx0, tx1);
+      float ty0 = dy_r * (cc[1] - child_size_half) + mpy, ty1 = dy_r * (cc[1] + child_size_half) + mpy;
+      swap_to_increasing(ty0, ty1);
+      float tz0 = dz_r * (cc[2] - child_size_half) + mpz, tz1 = dz_r * (cc[2] + child_size_half) + mpz;
+      swap_to_increasing(tz0, tz1);
+      const float tc_min = cl_max(tx0, cl_max(ty0, tz0));
+      const float tc_max = cl_min(tx1, cl_min(ty1, tz1));
+      {
+        const float t0[3] = {tx0, ty0, tz0}, t1[3] = {tx1, ty1, tz1};
+        if (near_edge(t0, t1, tc_min, tc_max)) r->boundary = true;
+      }
+      if (parent->parent_node_index >= n) return;
+      const Node pn = nodes[parent->parent_node_index];
+      if (valid_flag(pn.masks, idx)) {
+        const uint32_t child = nth_child_idx(pn.masks, pn.first, idx);  // absolute
+        if (leaf_flag(pn.masks, idx) || ratio * tc_max > scale_exp2) {
+          r->hit = true;
+          r->node = child;
+          r->depth = scale_max - scale;
+          r->t = parent->parent_tmin;
+          r->loops = while_counter;
+          std::memcpy(r->center, cc, sizeof cc);
+          return;
+        }
+        parent->parent_tmin = tc_max;
+        --scale;
+        stack[scale] = StackElement{child, tc_min, tc_max, {cc[0], cc[1], cc[2]}};
+        ++r->pushes;
+        continue;
+      } else {
+        parent->parent_tmin = tc_max;
+      }
+    } else {
+      ++scale;
+      continue;
+    }
+  }
+  r->hit = false;
+  r->node = 0xFFFFFFFFu;
+  r->t = t_max;
+  r->loops = while_counter;
+}
+
+// Single-SVO files, read independently of the library (own parser).
+struct oracle_single {
+  std::vector<Node> nodes;        // absolute first-child indices, as stored
+  std::vector<uint32_t> aidx;     // attribute index per node
+  std::vector<uint32_t> packages; // {normal, color} per package
+  uint32_t depth = 0;
+};
+
+int read_single(const std::string& base, oracle_single* s) {
+  FILE* f = fopen((base + ".svo").c_str(), "rb");
+  if (!f) return 2;
+  std::vector<std::string> words;
+  bool v1 = false;
+  for (int k = 0; k < 3; ++k) {
+    std::string w;
+    int c;
+    while ((c = fgetc(f)) == ' ') {}
+    while (c != EOF && c != ' ') { w.push_back((char)c); c = fgetc(f); }
+    if (c == EOF) { fclose(f); return 3; }
+    words.push_back(w);
+    if (k == 0) v1 = (w == "v1");
+    if (k == 1 && !v1) break;
+  }
+  const uint32_t n = (uint32_t)std::strtoul(words.back().c_str(), nullptr, 10);
+  s->depth = (uint32_t)std::strtoul(words[words.size() - 2].c_str(), nullptr, 10);
+  s->nodes.resize(n);
+  s->aidx.resize(n);
+  bool ok = fread(s->nodes.data(), 8, n, f) == n;
+  if (v1) ok = ok && fread(s->aidx.data(), 4, n, f) == n;
+  else for (uint32_t i = 0; i < n; ++i) s->aidx[i] = i;
+  fclose(f);
+  if (!ok) return 3;
+  f = fopen((base + ".ia").c_str(), "rb");
+  if (!f) return 2;
+  uint32_t blocks = 0, np = 0, w = 0;
+  ok = read_u32(f, &blocks) && read_u32(f, &np) && read_u32(f, &w) && blocks == 1 && w == 2;
+  if (ok) {
+    s->packages.resize(2 * (size_t)np);
+    ok = fread(s->packages.data(), 4, s->packages.size(), f) == s->packages.size();
+  }
+  fclose(f);
+  if (!ok) return 3;
+  for (uint32_t i = 0; i < n; ++i)
+    if (s->aidx[i] >= np) return 3;
+  return 0;
+}
+
 }  // namespace
 
 // ------------------------------------------------------------ C interface
@@ -660,5 +800,46 @@ int oracle_ray_dirs(const void* camera, float* dirs) {
 }
 
 int oracle_max_threads(void) { return omp_get_max_threads(); }
+
+int oracle_open_single(const char* base, oracle_single** out) {
+  if (!base || !out) return 1;
+  oracle_single* s = new oracle_single();
+  const int r = read_single(base, s);
+  if (r) { delete s; return r; }
+  *out = s;
+  return 0;
+}
+
+void oracle_free_single(oracle_single* s) { delete s; }
+
+// Single-SVO frame with policy B (fillFrameBuffer_rgba.cl::renderToBuffer
+// mode 0): colour = packages[attribIndices[node]] (:640), alpha forced,
+// treelet id 0 for a hit (the whole SVO is one treelet).
+int oracle_render_single(const oracle_single* s, const void* camera, float* t, uint32_t* node, uint32_t* treelet,
+                         uint32_t* rgba, uint8_t* boundary, uint32_t* pushes, int threads) {
+  if (!s || !camera) return 1;
+  Camera cam;
+  std::memcpy(&cam, camera, sizeof cam);
+  const int W = (int)cam.width, H = (int)cam.height;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads > 0 ? threads : 1)
+  for (int y = 0; y < H; ++y) {
+    for (int x = 0; x < W; ++x) {
+      float d[3];
+      pixel_ray(cam, (uint32_t)x, (uint32_t)y, d);
+      const float o[3] = {cam.position[0], cam.position[1], cam.position[2]};
+      Result r;
+      sample_fill(s->nodes.data(), s->nodes.size(), o, d, cam.t_scale_ratio, &r);
+      const size_t p = (size_t)y * W + x;
+      const bool hit = r.hit && r.node < s->nodes.size();
+      if (t) t[p] = r.t;
+      if (node) node[p] = hit ? r.node : 0xFFFFFFFFu;
+      if (treelet) treelet[p] = hit ? 0u : 0xFFFFFFFFu;
+      if (rgba) rgba[p] = hit ? (s->packages[2 * (size_t)s->aidx[r.node] + 1] | 0xFFu) : 0x334D33FFu;
+      if (boundary) boundary[p] = r.boundary ? 1 : 0;
+      if (pushes) pushes[p] = r.pushes;
+    }
+  }
+  return 0;
+}
 
 }  // extern "C"
