@@ -198,8 +198,31 @@ typedef struct svo_camera {
   float t_scale_ratio; /* LOD: frameBufferSize.z; 0 disables the LOD stop */
   uint32_t width;
   uint32_t height;
-  uint32_t reserved;
+  uint32_t view_mode;  /* renderToBuffer's otherParams.x, SVO_VIEW_*: what
+                          svo_frame_out.shade holds (other outputs are the
+                          same in every mode) */
 } svo_camera;
+
+/* View modes of renderToBuffer (opencl/treeletRenderer_rgba.cl:644-732),
+ * written as FP32 RGBA to svo_frame_out.shade:
+ *   COLOR      colour of the hit node (alpha 1), miss 0.2,0.3,0.2,1
+ *   NORMAL     unpacked normal of the hit node, miss 0.2,0.3,0.2,1
+ *   ITERATIONS loop count through the wavelength colour map
+ *              (DataPointToColor(loops, 0, 390), :488-596, pow in FP32)
+ *   SVO_DEPTH  depth/13 on a hit, 0 on a miss (shade_svoDepth)
+ *   T          1 - 0.1 t (shade_pixelDepth)
+ *   LOOPS      1 - loops/200 (shade_iterationDepth)
+ *   PHONG      shade_phong (:452-510): diffuse + specular towards a light
+ *              at (-120,0,0), smoothstep, no shadow ray; miss 0.4,0.6,0.4,1
+ *   QUALITY    sqrt(sqrt(quality)) with quality = scale_exp2 -
+ *              tScaleRatio*tcMin at a leaf, 0 at an LOD stop; miss
+ *              0.6,0.3,0.3,1
+ * The reference's FP64 literals in the phong and colour-map code are taken
+ * as FP32 (that code does not build as OpenCL C without double support). */
+enum {
+  SVO_VIEW_COLOR = 0, SVO_VIEW_NORMAL = 1, SVO_VIEW_ITERATIONS = 2, SVO_VIEW_SVO_DEPTH = 3,
+  SVO_VIEW_T = 4, SVO_VIEW_LOOPS = 5, SVO_VIEW_PHONG = 6, SVO_VIEW_QUALITY = 7
+};
 
 /* Pinhole camera at `eye` looking at `target` with `up`, vertical field of
  * view fov_y_deg, far plane `far_dist`; fills the FP32 uniforms (computed in
@@ -307,13 +330,15 @@ enum { SVO_POLICY_TREELET = 0, SVO_POLICY_SINGLE = 1, SVO_POLICY_CPU3 = 2 };
  *             (r<<24|g<<16|b<<8|a); miss = background 0.2,0.3,0.2,1
  *             -> 0x334D33FF
  *   pushes  : number of stack pushes of the ray (traversal statistics;
- *             8 bytes of node data are fetched per push)                */
+ *             8 bytes of node data are fetched per push)
+ *   shade   : 4 floats per pixel, the camera's view mode (SVO_VIEW_*)    */
 typedef struct svo_frame_out {
   float* t;
   uint32_t* node;
   uint32_t* treelet;
   uint32_t* rgba;
   uint32_t* pushes;
+  float* shade;
 } svo_frame_out;
 
 /* Renders n_cams frames in one launch on `stream` (asynchronous). */

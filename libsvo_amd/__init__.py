@@ -52,6 +52,14 @@ def device_count() -> int:
     return n.value
 
 
+def with_view_mode(cam: svo_camera, mode) -> svo_camera:
+    """Copy of `cam` with another view mode (name or SVO_VIEW_* number)."""
+    c = svo_camera()
+    ctypes.pointer(c)[0] = cam
+    c.view_mode = _capi.VIEW_MODES[mode] if isinstance(mode, str) else int(mode)
+    return c
+
+
 def look_at(eye, target, up=(0.0, 1.0, 0.0), fov_y_deg: float = 60.0, width: int = 512, height: int = 512,
             far: float = 20.0, lod: float = 0.0) -> svo_camera:
     """Pinhole camera uniforms; ``lod`` is the reference's tScaleRatio
@@ -324,7 +332,8 @@ class TreeletMemoryManagerHip:
 
 @dataclass
 class Frame:
-    """Device outputs of one camera (width*height each)."""
+    """Device outputs of one camera (width*height each; shade 4 floats per
+    pixel)."""
     width: int
     height: int
     t: DeviceBuffer
@@ -332,16 +341,18 @@ class Frame:
     treelet: DeviceBuffer
     rgba: DeviceBuffer
     pushes: Optional[DeviceBuffer] = None
+    shade: Optional[DeviceBuffer] = None
 
     @classmethod
-    def alloc(cls, width: int, height: int, pushes: bool = False) -> "Frame":
+    def alloc(cls, width: int, height: int, pushes: bool = False, shade: bool = False) -> "Frame":
         n = width * height
         return cls(width, height, DeviceBuffer(4 * n), DeviceBuffer(4 * n), DeviceBuffer(4 * n), DeviceBuffer(4 * n),
-                   DeviceBuffer(4 * n) if pushes else None)
+                   DeviceBuffer(4 * n) if pushes else None, DeviceBuffer(16 * n) if shade else None)
 
     def out(self) -> svo_frame_out:
         return svo_frame_out(self.t.ptr, self.node.ptr, self.treelet.ptr, self.rgba.ptr,
-                             self.pushes.ptr if self.pushes is not None else None)
+                             self.pushes.ptr if self.pushes is not None else None,
+                             self.shade.ptr if self.shade is not None else None)
 
     def fetch(self) -> Dict[str, np.ndarray]:
         shp = (self.height, self.width)
@@ -350,6 +361,8 @@ class Frame:
                "rgba": self.rgba.to_numpy(np.uint32).reshape(shp)}
         if self.pushes is not None:
             out["pushes"] = self.pushes.to_numpy(np.uint32).reshape(shp)
+        if self.shade is not None:
+            out["shade"] = self.shade.to_numpy(np.float32).reshape(self.height, self.width, 4)
         return out
 
 
@@ -380,10 +393,11 @@ def render_timed(mm: TreeletMemoryManagerHip, cams: Sequence[svo_camera], frames
     return ms.value
 
 
-def render(mm: TreeletMemoryManagerHip, cam: svo_camera, policy="treelet", pushes: bool = True) -> Dict[str, np.ndarray]:
+def render(mm: TreeletMemoryManagerHip, cam: svo_camera, policy="treelet", pushes: bool = True,
+           shade: bool = False) -> Dict[str, np.ndarray]:
     """Renders one camera and returns host copies of t, node, treelet, rgba
-    (and the per-ray push counts)."""
-    f = Frame.alloc(cam.width, cam.height, pushes)
+    (and the per-ray push counts, and the view-mode image when shade)."""
+    f = Frame.alloc(cam.width, cam.height, pushes, shade)
     render_to_buffer(mm, [cam], [f], policy)
     check(lib.svo_device_synchronize(), "svo_device_synchronize")
     return f.fetch()

@@ -24,6 +24,7 @@ SVO_POLICY_CPU3 = 2
 SVO_BUILD_PACK_BFS = 1
 SVO_SINGLE_SVOLOADER, SVO_SINGLE_V1 = 1, 2
 SVO_STACK_LDS, SVO_STACK_PRIVATE = 0, 1
+VIEW_MODES = {"color": 0, "normal": 1, "iterations": 2, "svo_depth": 3, "t": 4, "loops": 5, "phong": 6, "quality": 7}
 MISS = 0xFFFFFFFF
 BACKGROUND_RGBA = 0x334D33FF
 
@@ -45,7 +46,7 @@ class svo_scene_info(ctypes.Structure):
 class svo_camera(ctypes.Structure):
     _fields_ = [("far_lower_left", f32 * 4), ("pixel_h", f32 * 4), ("pixel_v", f32 * 4),
                 ("position", f32 * 4), ("t_scale_ratio", f32), ("width", u32), ("height", u32),
-                ("reserved", u32)]
+                ("view_mode", u32)]
 
 
 class svo_mm_info(ctypes.Structure):
@@ -59,7 +60,7 @@ class svo_upload_stats(ctypes.Structure):
 
 
 class svo_frame_out(ctypes.Structure):
-    _fields_ = [("t", vp), ("node", vp), ("treelet", vp), ("rgba", vp), ("pushes", vp)]
+    _fields_ = [("t", vp), ("node", vp), ("treelet", vp), ("rgba", vp), ("pushes", vp), ("shade", vp)]
 
 
 class svo_view(ctypes.Structure):

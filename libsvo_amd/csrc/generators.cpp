@@ -316,6 +316,6 @@ extern "C" svo_status svo_camera_look_at(const double eye[3], const double targe
   out->t_scale_ratio = (float)(lod_multiplier * th * aspect / (width * 0.5));
   out->width = width;
   out->height = height;
-  out->reserved = 0;
+  out->view_mode = SVO_VIEW_COLOR;
   return SVO_OK;
 }

@@ -569,6 +569,7 @@ svo_status mm_trace_pools(const svo_mm* mm, int policy, TracePools* P) {
   *P = TracePools{};
   P->nodes = mm->d_nodes;
   P->colors = mm->d_colors;
+  P->normals = mm->d_normals;
   P->slot_gid = mm->d_slot_gid;
   P->nodes_per_treelet = mm->N;
   P->slot_shift = ((mm->N & (mm->N - 1)) == 0) ? (uint32_t)__builtin_ctz(mm->N) : 32u;

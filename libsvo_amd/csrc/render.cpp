@@ -39,6 +39,7 @@ svo_status stage_views(const svo_view* views, uint32_t n, int policy, const Laun
   out->frames.resize(n);
   uint64_t tiles = 0;
   uint32_t levels = 1;
+  bool shade = false;
   for (uint32_t i = 0; i < n; ++i) {
     const svo_view& v = views[i];
     if (!v.mm) { set_error("view %u has no residency handle", i); return SVO_ERR_ARG; }
@@ -55,15 +56,18 @@ svo_status stage_views(const svo_view* views, uint32_t n, int policy, const Laun
     F.out = v.out;
     F.tile_begin = (uint32_t)tiles;
     F.tiles_x = (c.width + tw - 1) / tw;
+    if (c.view_mode > SVO_VIEW_QUALITY) { set_error("view %u: unknown view mode %u", i, c.view_mode); return SVO_ERR_ARG; }
     if (v.feedback_stride > 65535u) { set_error("view %u: feedback stride %u too large", i, v.feedback_stride); return SVO_ERR_ARG; }
     F.fb_stride = v.feedback_stride;
     F.pad = 0;
     tiles += (uint64_t)F.tiles_x * ((c.height + th - 1) / th);
     levels = std::max(levels, F.pools.stack_levels);
+    shade = shade || v.out.shade != nullptr;
   }
   if (tiles > 0x7FFFFFFFull) { set_error("launch of %llu tiles too large", (unsigned long long)tiles); return SVO_ERR_ARG; }
   out->num_tiles = (uint32_t)tiles;
   out->stack_levels = levels;
+  out->shade = shade;
   return SVO_OK;
 }
 
@@ -141,7 +145,7 @@ struct svo_renderer {
   }
 
   svo_status launch(const LaunchShape& sh, const StagedLaunch& sg, Slot* s, int policy, hipStream_t st) {
-    int e = launch_trace(sh, s->d, (uint32_t)sg.frames.size(), sg.num_tiles, sg.stack_levels, policy, st);
+    int e = launch_trace(sh, s->d, (uint32_t)sg.frames.size(), sg.num_tiles, sg.stack_levels, policy, sg.shade, st);
     if (e != 0) { set_error("trace launch failed: %s", hipGetErrorString((hipError_t)e)); return SVO_ERR_HIP; }
     HIP_TRY(hipEventRecord(s->done, st));
     s->pending = true;
@@ -299,7 +303,7 @@ extern "C" svo_status svo_batch_create(svo_renderer* r, const svo_view* views, u
   HIP_TRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
   hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
   if (e == hipSuccess) {
-    int le = launch_trace(b->shape, b->d, n, b->sg.num_tiles, b->sg.stack_levels, policy, cap);
+    int le = launch_trace(b->shape, b->d, n, b->sg.num_tiles, b->sg.stack_levels, policy, b->sg.shade, cap);
     hipError_t e2 = hipStreamEndCapture(cap, &b->graph);
     e = le != 0 ? (hipError_t)le : e2;
   }

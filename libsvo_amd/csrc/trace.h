@@ -15,6 +15,7 @@ namespace svo_impl {
 struct TracePools {
   const uint64_t* nodes;
   const uint32_t* colors;
+  const uint32_t* normals;
   const int32_t* slot_gid;
   uint32_t nodes_per_treelet;
   uint32_t slot_shift;      // log2(nodes_per_treelet) when a power of two, else 32
@@ -62,9 +63,10 @@ bool shape_supported(const LaunchShape& s);
 
 // Enqueues the traversal of `num_tiles` tiles spread over `num_frames` views
 // (frames_dev in device memory) on `stream`; `stack_levels` is the largest
-// stack depth over the views.  Returns a hipError_t value (0 = success).
+// stack depth over the views; `shade` adds the view-mode pass for views
+// with a shade output.  Returns a hipError_t value (0 = success).
 int launch_trace(const LaunchShape& shape, const TraceFrame* frames_dev, uint32_t num_frames, uint32_t num_tiles,
-                 uint32_t stack_levels, int policy, void* stream);
+                 uint32_t stack_levels, int policy, bool shade, void* stream);
 
 // Host image of one launch's view descriptors (render.cpp): views in order,
 // tiles numbered consecutively for the given shape.
@@ -72,6 +74,7 @@ struct StagedLaunch {
   std::vector<TraceFrame> frames;
   uint32_t num_tiles = 0;
   uint32_t stack_levels = 1;
+  bool shade = false;  // some view has a view-mode output
 };
 svo_status stage_views(const svo_view* views, uint32_t n, int policy, const LaunchShape& shape, int device,
                        StagedLaunch* out);

@@ -130,6 +130,12 @@ struct Hit {
   // kind 0 = no record (miss or maxLoops).
   uint32_t kind;
   float err;
+  // SampleResult fields of the view modes (treeletRenderer_rgba.cl:39-48)
+  uint32_t depth;    // scaleMax - scale of the hit (+1 at a policy-A LOD stop), 0 on a miss
+  uint32_t loops;    // numWhileLoops
+  float center[3];   // nodeCenter: centre of the hit node (the parent's at maxLoops)
+  float quality;     // scale_exp2 - tScaleRatio*tcMin at a leaf, 0 at an LOD stop,
+                     // scale_exp2 - tScaleRatio*parentTMin at a policy-A maxLoops
 };
 
 __device__ __forceinline__ float leaf_error(float se, float ratio, float tc_min) { return se - (ratio * tc_min); }
@@ -191,7 +197,7 @@ struct RayStack<SVO_STACK_PRIVATE, BLOCK> {  // private (scratch-backed) arrays
 // chains of a wave.)
 template <int POLICY, typename Stack>
 __device__ Hit traverse(const TracePools& S, const float o[3], float d[3], float ratio, Stack& stk) {
-  Hit r{0xFFFFFFFFu, __int_as_float(0x7F800000), 0u, 0u, 0.0f};
+  Hit r{0xFFFFFFFFu, __int_as_float(0x7F800000), 0u, 0u, 0.0f, 0u, 0u, {0.0f, 0.0f, 0.0f}, 0.0f};
   float t_min, t_max;
   if (!root_box(o, d, t_min, t_max)) return r;
   r.t = t_max;
@@ -214,44 +220,45 @@ __device__ Hit traverse(const TracePools& S, const float o[3], float d[3], float
   uint64_t nd = S.nodes[0];
   uint64_t path = 0;
   uint32_t iter = 0;
+  uint32_t pushes = 0;
   // Cycle detection.  The traversal is deterministic and its whole state is
   // (scale, path, tmin, tmax): the path fixes the node and its ancestors, and
   // every other quantity is recomputed from them.  Some rays fall into an
   // exact cycle (an advance that leaves tmin unchanged, or a push/pop loop
   // around a child that is re-entered at the same t) and spin until maxLoops,
   // which in the reference returns the parent as a hit (policy A) or a miss
-  // (policy C).  A snapshot taken at every power-of-two iteration >= 128 is
-  // compared with later states; on a repeat with period p the counter jumps
-  // whole periods towards maxLoops, so the ray reaches maxLoops in the same
-  // state as the reference (pushes are credited per skipped period).
+  // (policies B, C).  A snapshot taken at every power-of-two iteration >= 128
+  // is compared with later states; on a repeat with period p the counter
+  // jumps whole periods towards maxLoops, so the ray reaches maxLoops in the
+  // same state as the reference (pushes are credited per skipped period).
   uint64_t snap_path = 0;
   uint32_t snap_iter = 0, snap_scale = 0, snap_tmin = 0, snap_tmax = 0, snap_pushes = 0;
+  // outcome: 0 running, 1 hit (leaf / LOD stop), 2 miss, 3 maxLoops, 4 corrupt offset
+  uint32_t outcome = 0, hit_node = 0, hit_kind = 0, hit_ci = 0;
+  float hit_tcmin = 0.0f;
 
-  for (;;) {
+  do {
     const float se = exp2i(scale - smax);  // scale_exp2
     const float h = se * 0.5f;             // childSizeHalf
     ++iter;
     if (iter >= 128u) {
       if ((iter & (iter - 1u)) == 0u) {
         snap_iter = iter; snap_scale = (uint32_t)scale; snap_path = path;
-        snap_tmin = __float_as_uint(tmin); snap_tmax = __float_as_uint(tmax); snap_pushes = r.pushes;
+        snap_tmin = __float_as_uint(tmin); snap_tmax = __float_as_uint(tmax); snap_pushes = pushes;
       } else if (snap_iter != 0u && (uint32_t)scale == snap_scale && path == snap_path &&
                  __float_as_uint(tmin) == snap_tmin && __float_as_uint(tmax) == snap_tmax &&
                  iter < S.max_loops) {
         // the state repeats with period p: jump whole periods towards maxLoops
         const uint32_t p = iter - snap_iter;
         const uint32_t skip = (S.max_loops - iter) / p;
-        r.pushes += skip * (r.pushes - snap_pushes);
+        pushes += skip * (pushes - snap_pushes);
         iter += skip * p;
         snap_iter = 0u;
       }
     }
     if (iter == S.max_loops) {
-      if (POLICY == 0) {  // maxLoops returns the current parent as a hit
-        r.node = nidx;
-        r.t = tmin;
-      }
-      return r;
+      outcome = 3;
+      break;
     }
     const bool pop = policy_pop<POLICY>(tmin, tmax);
 
@@ -281,45 +288,38 @@ __device__ Hit traverse(const TracePools& S, const float o[3], float d[3], float
 
       const uint32_t masks = (uint32_t)(nd >> 32);
       if ((masks >> (8u + ci)) & 1u) {
-        const uint32_t child = nidx + (uint32_t)nd + (uint32_t)__popc((masks >> 8) & ((1u << ci) - 1u));  // relative
-        if ((masks >> ci) & 1u) {  // leaf
-          r.node = child;
-          r.t = tmin;
-          r.kind = 1u;
-          r.err = leaf_error(se, ratio, tc_min);
-          return r;
-        }
+        const uint32_t child = nidx + (uint32_t)nd + (uint32_t)__popc((masks >> 8) & ((1u << ci) - 1u));
+        const bool is_leaf = (masks >> ci) & 1u;
         bool stop;
         if (POLICY == 0) stop = (ratio * tc_min > se) || (S.lod_cap != 0 && (uint32_t)(smax - scale) == S.lod_cap);
         else if (POLICY == 1) stop = ratio * tc_max > se;
         else stop = (ratio * tc_max > se) || scale == 0;
-        if (stop) {
-          r.node = child;
-          r.t = tmin;
-          r.kind = 2u;
-          r.err = stop_error(se, ratio, tc_min);
-          return r;
+        if (is_leaf || stop) {
+          outcome = 1;
+          hit_node = child;
+          hit_kind = is_leaf ? 1u : 2u;
+          hit_ci = ci;
+          hit_tcmin = tc_min;
+        } else if (child >= S.pool_nodes) {  // corrupt offset: never read outside the pool
+          outcome = 4;
+        } else {  // push
+          ++pushes;
+          stk.put(scale - mps, nidx);
+          path = (path << 3) | ci;
+          --scale;
+          c[0] = cc[0]; c[1] = cc[1]; c[2] = cc[2];
+          tmin = tc_min;
+          tmax = tc_max;
+          fidx = child;
+          fetch = true;
         }
-        if (child >= S.pool_nodes) {  // corrupt offset: never read outside the pool
-          r.node = 0xFFFFFFFFu;
-          return r;
-        }
-        // push
-        ++r.pushes;
-        stk.put(scale - mps, nidx);
-        path = (path << 3) | ci;
-        --scale;
-        c[0] = cc[0]; c[1] = cc[1]; c[2] = cc[2];
-        tmin = tc_min;
-        tmax = tc_max;
-        fidx = child;
-        fetch = true;
       } else {
         tmin = tc_max;  // advance
       }
+    } else if (scale + 1 >= smax) {
+      outcome = 2;  // left the root: miss, t = root tMax
     } else {
       ++scale;
-      if (scale >= smax) return r;  // left the root: miss, t = root tMax
       const float t_child = tmax;
       fidx = stk.get(scale - mps);
       fetch = true;
@@ -336,9 +336,30 @@ __device__ Hit traverse(const TracePools& S, const float o[3], float d[3], float
       nidx = fidx;
       nd = S.nodes[fidx];
     }
-  }
-}
+  } while (outcome == 0);
 
+  r.pushes = pushes;
+  r.loops = iter;
+  const float se = exp2i(scale - smax);
+  if (outcome == 1) {
+    r.node = hit_node;
+    r.t = tmin;
+    r.kind = hit_kind;
+    r.err = hit_kind == 1u ? leaf_error(se, ratio, hit_tcmin) : stop_error(se, ratio, hit_tcmin);
+    r.depth = (uint32_t)(smax - scale) + ((POLICY == 0 && hit_kind == 2u) ? 1u : 0u);
+    r.center[0] = (-0.5f + (float)((hit_ci >> 2) & 1u)) * se + c[0];
+    r.center[1] = (-0.5f + (float)((hit_ci >> 1) & 1u)) * se + c[1];
+    r.center[2] = (-0.5f + (float)(hit_ci & 1u)) * se + c[2];
+    r.quality = hit_kind == 1u ? r.err : 0.0f;
+  } else if (outcome == 3 && POLICY == 0) {  // maxLoops returns the current parent as a hit
+    r.node = nidx;
+    r.t = tmin;
+    r.depth = (uint32_t)(smax - scale);
+    r.center[0] = c[0]; r.center[1] = c[1]; r.center[2] = c[2];
+    r.quality = se - (ratio * tmin);
+  }
+  return r;
+}
 
 // Wave-uniform traversal parameters, held in scalar registers.
 struct Uniform {
@@ -406,6 +427,7 @@ __device__ Hit traverse_fast(const Uniform& U, const float o[3], const float d[3
   uint32_t outcome = 0;
   uint32_t hit_node = 0;
   uint32_t hit_kind = 0;
+  uint32_t hit_ci = 0;
   float hit_err = 0.0f;
 
   do {
@@ -466,6 +488,7 @@ __device__ Hit traverse_fast(const Uniform& U, const float o[3], const float d[3
           hit_node = child;
           const bool is_leaf = (masks >> ci) & 1u;
           hit_kind = is_leaf ? 1u : 2u;
+          hit_ci = ci;
           hit_err = is_leaf ? leaf_error(se, ratio, tc_min) : stop_error(se, ratio, tc_min);
         } else if (child >= U.pool_nodes) {
           outcome = 4;
@@ -509,15 +532,25 @@ __device__ Hit traverse_fast(const Uniform& U, const float o[3], const float d[3
   } while (outcome == 0);
 
   r.pushes = pushes;
+  r.loops = iter;
+  const float se = exp2i(scale - U.smax);
   if (outcome == 1) {
     r.node = hit_node;
     r.t = tmin;
     r.kind = hit_kind;
     r.err = hit_err;
+    r.depth = (uint32_t)(U.smax - scale) + ((POLICY == 0 && hit_kind == 2u) ? 1u : 0u);
+    const float h = se * 0.5f;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) r.center[a] = c[a] + (((hit_ci >> (2 - a)) & 1u) ? h : -h);
+    r.quality = hit_kind == 1u ? hit_err : 0.0f;
   } else if (outcome == 3) {
     if (POLICY == 0) {  // maxLoops returns the current parent as a hit
       r.node = nidx;
       r.t = tmin;
+      r.depth = (uint32_t)(U.smax - scale);
+      r.center[0] = c[0]; r.center[1] = c[1]; r.center[2] = c[2];
+      r.quality = se - (ratio * tmin);
     }
   }
   return r;
@@ -528,7 +561,7 @@ __device__ Hit traverse_fast(const Uniform& U, const float o[3], const float d[3
 // generic loop.
 template <int POLICY, typename Stack>
 __device__ Hit trace_ray(const TracePools& S, const float o[3], float d[3], float ratio, Stack& stk) {
-  Hit r{0xFFFFFFFFu, __int_as_float(0x7F800000), 0u, 0u, 0.0f};
+  Hit r{0xFFFFFFFFu, __int_as_float(0x7F800000), 0u, 0u, 0.0f, 0u, 0u, {0.0f, 0.0f, 0.0f}, 0.0f};
   float t_min, t_max;
   if (!root_box(o, d, t_min, t_max)) return r;
   float dc[3] = {d[0], d[1], d[2]};
@@ -576,6 +609,116 @@ __device__ __forceinline__ void record_feedback(const TracePools& S, const Hit& 
   }
 }
 
+// ------------------------------------------------------------ view modes
+// renderToBuffer's modes (treeletRenderer_rgba.cl:644-732) as FP32 RGBA.
+// OpenCL vector arithmetic is per component with the same rounding; dot()
+// is (x*x + y*y) + z*z and normalize(v) is v / sqrt(dot(v, v)), as in the
+// ray generation; max(0, x) is OpenCL's (x if 0 < x, else 0).
+__device__ __forceinline__ float4 unpack_rgba(uint32_t v) {  // unpackRgbaFromUnsigned (:183-195)
+  return make_float4((float)((v >> 24) & 255u), (float)((v >> 16) & 255u), (float)((v >> 8) & 255u), (float)(v & 255u));
+}
+__device__ __forceinline__ float3 get_normal(uint32_t packed) {  // getNormal (:389-393)
+  const float4 u = unpack_rgba(packed);
+  const float k = 0.003921569f;
+  return make_float3(((u.x * k) - 0.5f) * 2.0f, ((u.y * k) - 0.5f) * 2.0f, ((u.z * k) - 0.5f) * 2.0f);
+}
+__device__ __forceinline__ float4 get_color(uint32_t packed) {  // getColor (:399-404)
+  const float4 u = unpack_rgba(packed);
+  const float k = 0.003921569f;
+  return make_float4(u.x * k, u.y * k, u.z * k, u.w * k);
+}
+__device__ __forceinline__ float dot3(float3 a, float3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+__device__ __forceinline__ float3 normalize3(float3 v) {
+  const float l = sqrtf(dot3(v, v));
+  return make_float3(v.x / l, v.y / l, v.z / l);
+}
+__device__ __forceinline__ float smoothstep01(float x) {  // OpenCL smoothstep(0, 1, x)
+  const float t = fminf(fmaxf((x - 0.0f) / (1.0f - 0.0f), 0.0f), 1.0f);
+  return t * t * (3.0f - 2.0f * t);
+}
+// Adjust (:518-529): round(255 * pow(c*f, 0.8)) clamped to [0, 255]
+__device__ __forceinline__ float adjust(float c, float f) {
+  if (c == 0.0f) return 0.0f;
+  const int r = (int)roundf(255.0f * powf(c * f, 0.8f));
+  return (float)min(255, max(0, r));
+}
+// DataPointToColor (:580-596) over WavelengthToRGB (:533-576)
+__device__ float3 data_point_color(float value, float vmin, float vmax) {
+  const float w = (value - vmin) / (vmax - vmin) * (780.0f - 380.0f) + 380.0f;
+  float r, g, b, f;
+  if (380.0f <= w && w <= 440.0f) { r = -(w - 440.0f) / (440.0f - 380.0f); g = 0.0f; b = 1.0f; }
+  else if (440.0f < w && w <= 490.0f) { r = 0.0f; g = (w - 440.0f) / (490.0f - 440.0f); b = 1.0f; }
+  else if (490.0f < w && w <= 510.0f) { r = 0.0f; g = 1.0f; b = -(w - 510.0f) / (510.0f - 490.0f); }
+  else if (510.0f < w && w <= 580.0f) { r = (w - 510.0f) / (580.0f - 510.0f); g = 1.0f; b = 0.0f; }
+  else if (580.0f < w && w <= 645.0f) { r = 1.0f; g = -(w - 645.0f) / (645.0f - 580.0f); b = 0.0f; }
+  else if (645.0f < w && w <= 780.0f) { r = 1.0f; g = 0.0f; b = 0.0f; }
+  else { r = 0.0f; g = 0.0f; b = 0.0f; }
+  if (380.0f <= w && w <= 420.0f) f = 0.3f + 0.7f * (w - 380.0f) / (420.0f - 380.0f);
+  else if (420.0f < w && w <= 701.0f) f = 1.0f;
+  else if (701.0f < w && w <= 780.0f) f = 0.3f + 0.7f * (780.0f - w) / (780.0f - 701.0f);
+  else f = 0.0f;
+  const float fac = 1.0f / 255.0f;
+  return make_float3(adjust(r, f) * fac, adjust(g, f) * fac, adjust(b, f) * fac);
+}
+
+__device__ float4 shade_pixel(uint32_t mode, const Hit& hit, const float d[3], const TracePools& S) {
+  const bool hit_ok = hit.node != 0xFFFFFFFFu && hit.node < S.pool_nodes;
+  switch (mode) {
+    case SVO_VIEW_NORMAL:
+      if (hit_ok) {
+        const float3 n = get_normal(S.normals[hit.node]);
+        return make_float4(n.x, n.y, n.z, 1.0f);
+      }
+      return make_float4(0.2f, 0.3f, 0.2f, 1.0f);
+    case SVO_VIEW_ITERATIONS: {
+      const float3 c = data_point_color((float)hit.loops, 0.0f, (float)(13 * 30));
+      return make_float4(c.x, c.y, c.z, 1.0f);
+    }
+    case SVO_VIEW_SVO_DEPTH: {
+      const float v = (float)hit.depth / (float)13;
+      const float h = hit_ok ? 1.0f : 0.0f;
+      return make_float4(v * h, v * h, v * h, 1.0f);
+    }
+    case SVO_VIEW_T: {
+      const float v = 1.0f - hit.t * 0.1f;
+      return make_float4(v, v, v, 1.0f);
+    }
+    case SVO_VIEW_LOOPS: {
+      const float v = 1.0f - (float)hit.loops / 200.0f;
+      return make_float4(v, v, v, 1.0f);
+    }
+    case SVO_VIEW_PHONG: {
+      if (!hit_ok) return make_float4(0.4f, 0.6f, 0.4f, 1.0f);
+      const float3 n = normalize3(get_normal(S.normals[hit.node]));
+      float4 col = get_color(S.colors[hit.node]);
+      const float3 l = normalize3(make_float3(-120.0f - hit.center[0], 0.0f - hit.center[1], 0.0f - hit.center[2]));
+      const float ndl = cl_max(0.0f, dot3(n, l));
+      const float3 rf = normalize3(make_float3(((2.0f * n.x) * ndl) - l.x, ((2.0f * n.y) * ndl) - l.y,
+                                               ((2.0f * n.z) * ndl) - l.z));
+      const float rdv = cl_max(0.0f, dot3(rf, make_float3(-d[0], -d[1], -d[2])));
+      const float spec = cl_max(0.0f, powf(rdv, 10.0f)) * (ndl > 0.0f ? 1.0f : 0.0f);
+      const float shadow = 1.0f - 0.0f;
+      float4 o;
+      o.x = col.x * 0.5f + ((col.x * ndl) * shadow + (spec * shadow) * 0.2f) * 0.5f;
+      o.y = col.y * 0.5f + ((col.y * ndl) * shadow + (spec * shadow) * 0.2f) * 0.5f;
+      o.z = col.z * 0.5f + ((col.z * ndl) * shadow + (spec * shadow) * 0.2f) * 0.5f;
+      return make_float4(smoothstep01(o.x), smoothstep01(o.y), smoothstep01(o.z), 1.0f);
+    }
+    case SVO_VIEW_QUALITY:
+      if (hit_ok) {
+        const float q = sqrtf(sqrtf(hit.quality));
+        return make_float4(q, q, q, 1.0f);
+      }
+      return make_float4(0.6f, 0.3f, 0.3f, 1.0f);
+    default:  // SVO_VIEW_COLOR
+      if (hit_ok) {
+        const float4 c = get_color(S.colors[hit.node]);
+        return make_float4(c.x, c.y, c.z, 1.0f);
+      }
+      return make_float4(0.2f, 0.3f, 0.2f, 1.0f);
+  }
+}
+
 template <int POLICY, int TW, int WAVES, int STACK>
 __global__ __launch_bounds__(64 * WAVES) void trace_kernel(const TraceFrame* __restrict__ frames, uint32_t num_frames,
                                                            uint32_t num_tiles) {
@@ -609,7 +752,8 @@ __global__ __launch_bounds__(64 * WAVES) void trace_kernel(const TraceFrame* __r
     stk.base = reinterpret_cast<uint2*>(s_stack);
     stk.tid = tid;
   }
-  const Hit hit = trace_ray<POLICY>(S, o, d, F.cam.t_scale_ratio, stk);
+  float dt[3] = {d[0], d[1], d[2]};  // the traversal may clamp its copy
+  const Hit hit = trace_ray<POLICY>(S, o, dt, F.cam.t_scale_ratio, stk);
 
   const size_t pix = (size_t)y * F.cam.width + x;
   uint32_t tree = 0xFFFFFFFFu, rgba = 0x334D33FFu;
@@ -626,6 +770,62 @@ __global__ __launch_bounds__(64 * WAVES) void trace_kernel(const TraceFrame* __r
   if (F.out.treelet) F.out.treelet[pix] = tree;
   if (F.out.rgba) F.out.rgba[pix] = rgba;
   if (F.out.pushes) F.out.pushes[pix] = hit.pushes;
+  if (F.out.shade) {
+    // raw hit data for shade_kernel, which turns it into the view mode's
+    // colour in place (keeps the shading out of this kernel's registers)
+    uint4 raw;
+    raw.x = hit.node;
+    if (F.cam.view_mode == SVO_VIEW_PHONG) {
+      raw.y = __float_as_uint(hit.center[0]);
+      raw.z = __float_as_uint(hit.center[1]);
+      raw.w = __float_as_uint(hit.center[2]);
+    } else {
+      raw.y = __float_as_uint(hit.t);
+      raw.z = hit.depth | (hit.loops << 8);
+      raw.w = __float_as_uint(hit.quality);
+    }
+    reinterpret_cast<uint4*>(F.out.shade)[pix] = raw;
+  }
+}
+
+// Second pass of a launch with view-mode outputs: one lane per pixel of each
+// view whose shade output is set, tiles numbered as in trace_kernel.
+template <int TW>
+__global__ __launch_bounds__(64) void shade_kernel(const TraceFrame* __restrict__ frames, uint32_t num_frames,
+                                                   uint32_t num_tiles) {
+  constexpr uint32_t TH = 64u / TW;
+  const uint32_t tile = blockIdx.x;
+  const uint32_t lane = threadIdx.x;
+  if (tile >= num_tiles) return;
+  uint32_t lo = 0, hi = num_frames;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (frames[mid].tile_begin <= tile) lo = mid;
+    else hi = mid;
+  }
+  const TraceFrame& F = frames[lo];
+  if (!F.out.shade) return;
+  const uint32_t lt = tile - F.tile_begin;
+  const uint32_t x = (lt % F.tiles_x) * TW + (lane % TW);
+  const uint32_t y = (lt / F.tiles_x) * TH + (lane / TW);
+  if (x >= F.cam.width || y >= F.cam.height) return;
+  const size_t pix = (size_t)y * F.cam.width + x;
+  const uint4 raw = reinterpret_cast<const uint4*>(F.out.shade)[pix];
+  Hit hit{};
+  hit.node = raw.x;
+  if (F.cam.view_mode == SVO_VIEW_PHONG) {
+    hit.center[0] = __uint_as_float(raw.y);
+    hit.center[1] = __uint_as_float(raw.z);
+    hit.center[2] = __uint_as_float(raw.w);
+  } else {
+    hit.t = __uint_as_float(raw.y);
+    hit.depth = raw.z & 255u;
+    hit.loops = raw.z >> 8;
+    hit.quality = __uint_as_float(raw.w);
+  }
+  float d[3];
+  pixel_ray(F.cam, x, y, d);
+  reinterpret_cast<float4*>(F.out.shade)[pix] = shade_pixel(F.cam.view_mode, hit, d, F.pools);
 }
 
 __global__ void feedback_gather_kernel(const unsigned long long* __restrict__ fb_gid, const uint32_t* __restrict__ queue,
@@ -698,7 +898,7 @@ int launch_copy(void* dst, const void* src, size_t bytes, void* stream) {
 }
 
 int launch_trace(const LaunchShape& shape, const TraceFrame* frames_dev, uint32_t num_frames, uint32_t num_tiles,
-                 uint32_t stack_levels, int policy, void* stream) {
+                 uint32_t stack_levels, int policy, bool shade, void* stream) {
   if (num_tiles == 0) return 0;
   if (!shape_supported(shape) || stack_levels > (uint32_t)kMaxScale) return (int)hipErrorInvalidValue;
   const TraceFn fn = policy == SVO_POLICY_CPU3     ? pick_tile<2>(shape)
@@ -708,6 +908,12 @@ int launch_trace(const LaunchShape& shape, const TraceFrame* frames_dev, uint32_
   const dim3 grid((num_tiles + shape.waves - 1) / shape.waves);
   const size_t lds = shape.stack == SVO_STACK_LDS ? (size_t)stack_levels * block * sizeof(uint2) : 0;
   hipLaunchKernelGGL(fn, grid, dim3(block), lds, (hipStream_t)stream, frames_dev, num_frames, num_tiles);
+  int e = (int)hipGetLastError();
+  if (e != 0 || !shade) return e;
+  const dim3 sgrid(num_tiles), sblock(64);
+  if (shape.tile_w == 8) hipLaunchKernelGGL(shade_kernel<8>, sgrid, sblock, 0, (hipStream_t)stream, frames_dev, num_frames, num_tiles);
+  else if (shape.tile_w == 32) hipLaunchKernelGGL(shade_kernel<32>, sgrid, sblock, 0, (hipStream_t)stream, frames_dev, num_frames, num_tiles);
+  else hipLaunchKernelGGL(shade_kernel<16>, sgrid, sblock, 0, (hipStream_t)stream, frames_dev, num_frames, num_tiles);
   return (int)hipGetLastError();
 }
 

@@ -26,8 +26,8 @@ _lib.oracle_free.restype = None
 _lib.oracle_info.argtypes = [vp, ctypes.POINTER(u32), ctypes.POINTER(u32), ctypes.POINTER(u32)]
 _lib.oracle_make_incore.argtypes = [vp, u64]
 _lib.oracle_incore.argtypes = [vp] + [ctypes.POINTER(vp)] * 4 + [ctypes.POINTER(u64), ctypes.POINTER(u32)]
-_lib.oracle_render_buffers.argtypes = [vp, u64, vp, vp, u32, u32, u32, vp, ctypes.c_int] + [vp] * 12 + [ctypes.c_int]
-_lib.oracle_render.argtypes = [vp, vp, ctypes.c_int] + [vp] * 9 + [ctypes.c_int]
+_lib.oracle_render_buffers.argtypes = [vp, u64, vp, vp, u32, u32, u32, vp, ctypes.c_int] + [vp] * 14 + [ctypes.c_int]
+_lib.oracle_render.argtypes = [vp, vp, ctypes.c_int] + [vp] * 10 + [ctypes.c_int]
 _lib.oracle_trace_ray.argtypes = [vp, vp, vp, ctypes.c_float, ctypes.c_int, vp, vp, vp, vp, vp]
 _lib.oracle_max_threads.restype = ctypes.c_int
 _lib.oracle_ray_dirs.argtypes = [vp, vp]
@@ -35,7 +35,7 @@ _lib.oracle_ray_dirs.argtypes = [vp, vp]
 _lib.oracle_open_single.argtypes = [ctypes.c_char_p, ctypes.POINTER(vp)]
 _lib.oracle_free_single.argtypes = [vp]
 _lib.oracle_free_single.restype = None
-_lib.oracle_render_single.argtypes = [vp, vp] + [vp] * 6 + [ctypes.c_int]
+_lib.oracle_render_single.argtypes = [vp, vp] + [vp] * 7 + [ctypes.c_int]
 
 POLICY = {"treelet": 0, "single": 1, "cpu3": 2}
 
@@ -95,7 +95,8 @@ class OracleScene:
             "slot_gid": np.ctypeslib.as_array(ctypes.cast(ps, ctypes.POINTER(ctypes.c_int32)), (s,)).copy(),
         }
 
-    def render(self, cam, policy="treelet", threads: int = 0, extras: bool = True) -> Dict[str, np.ndarray]:
+    def render(self, cam, policy="treelet", threads: int = 0, extras: bool = True,
+               shade: bool = False) -> Dict[str, np.ndarray]:
         W, H = cam.width, cam.height
         out = {"t": np.empty((H, W), np.float32), "node": np.empty((H, W), np.uint32),
                "treelet": np.empty((H, W), np.uint32), "rgba": np.empty((H, W), np.uint32)}
@@ -103,10 +104,12 @@ class OracleScene:
             out.update({"boundary": np.empty((H, W), np.uint8), "pushes": np.empty((H, W), np.uint32),
                         "depth": np.empty((H, W), np.uint32), "loops": np.empty((H, W), np.uint32),
                         "center": np.empty((H, W, 3), np.float32)})
+        if shade:
+            out["shade"] = np.empty((H, W, 4), np.float32)
         g = out.get
         r = _lib.oracle_render(self._h, ctypes.byref(cam), POLICY.get(policy, policy), _ptr(g("t")), _ptr(g("node")),
                                _ptr(g("treelet")), _ptr(g("rgba")), _ptr(g("boundary")), _ptr(g("pushes")),
-                               _ptr(g("depth")), _ptr(g("loops")), _ptr(g("center")),
+                               _ptr(g("depth")), _ptr(g("loops")), _ptr(g("center")), _ptr(g("shade")),
                                threads if threads > 0 else max_threads())
         if r != 0:
             raise RuntimeError(f"oracle_render failed with {r}")
@@ -141,14 +144,16 @@ class OracleSingle:
             _lib.oracle_free_single(self._h)
             self._h = None
 
-    def render(self, cam, threads: int = 0) -> Dict[str, np.ndarray]:
+    def render(self, cam, threads: int = 0, shade: bool = False) -> Dict[str, np.ndarray]:
         W, H = cam.width, cam.height
         out = {"t": np.empty((H, W), np.float32), "node": np.empty((H, W), np.uint32),
                "treelet": np.empty((H, W), np.uint32), "rgba": np.empty((H, W), np.uint32),
                "boundary": np.empty((H, W), np.uint8), "pushes": np.empty((H, W), np.uint32)}
-        r = _lib.oracle_render_single(self._h, ctypes.byref(cam), *[_ptr(out[k]) for k in
+        if shade:
+            out["shade"] = np.empty((H, W, 4), np.float32)
+        r = _lib.oracle_render_single(self._h, ctypes.byref(cam), *[_ptr(out.get(k)) for k in
                                                                     ("t", "node", "treelet", "rgba", "boundary",
-                                                                     "pushes")],
+                                                                     "pushes", "shade")],
                                       threads if threads > 0 else max_threads())
         if r != 0:
             raise RuntimeError("oracle_render_single failed")
@@ -156,9 +161,11 @@ class OracleSingle:
 
 
 def render_buffers(nodes: np.ndarray, colors: np.ndarray, slot_gid: np.ndarray, nodes_per_treelet: int, depth: int,
-                   cam, policy="treelet", threads: int = 0, records: bool = False) -> Dict[str, np.ndarray]:
+                   cam, policy="treelet", threads: int = 0, records: bool = False,
+                   normals: Optional[np.ndarray] = None) -> Dict[str, np.ndarray]:
     """Oracle traversal over caller-provided incore arrays.  records=True adds
-    the per-ray analysis record (rec_kind, rec_err, rec_sub)."""
+    the per-ray analysis record (rec_kind, rec_err, rec_sub); normals (per
+    node) adds the view-mode image "shade" of the camera's view mode."""
     nodes = np.ascontiguousarray(nodes, np.uint32).reshape(-1, 2)
     colors = np.ascontiguousarray(colors, np.uint32)
     slot_gid = np.ascontiguousarray(slot_gid, np.int32)
@@ -169,13 +176,16 @@ def render_buffers(nodes: np.ndarray, colors: np.ndarray, slot_gid: np.ndarray, 
     if records:
         out.update({"rec_kind": np.empty((H, W), np.uint8), "rec_err": np.empty((H, W), np.float32),
                     "rec_sub": np.empty((H, W), np.uint32)})
+    if normals is not None:
+        normals = np.ascontiguousarray(normals, np.uint32)
+        out["shade"] = np.empty((H, W, 4), np.float32)
     g = out.get
     r = _lib.oracle_render_buffers(_ptr(nodes), nodes.shape[0], _ptr(colors), _ptr(slot_gid), nodes_per_treelet,
                                    slot_gid.size, depth, ctypes.byref(cam), POLICY.get(policy, policy),
                                    _ptr(out["t"]), _ptr(out["node"]), _ptr(out["treelet"]), _ptr(out["rgba"]),
                                    _ptr(out["boundary"]), _ptr(out["pushes"]), None, None, None,
                                    _ptr(g("rec_kind")), _ptr(g("rec_err")), _ptr(g("rec_sub")),
-                                   threads if threads > 0 else max_threads())
+                                   _ptr(normals), _ptr(g("shade")), threads if threads > 0 else max_threads())
     if r != 0:
         raise RuntimeError(f"oracle_render_buffers failed with {r}")
     return out

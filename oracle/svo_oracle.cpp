@@ -62,7 +62,7 @@ struct Camera {
   float pixel_v[4];
   float position[4];
   float t_scale_ratio;
-  uint32_t width, height, reserved;
+  uint32_t width, height, view_mode;
 };
 static_assert(sizeof(Camera) == 80, "camera layout");
 
@@ -194,6 +194,7 @@ struct Result {
   float center[3];
   uint32_t pushes;
   bool boundary;
+  float quality;  // SampleResult.quality (treeletRenderer_rgba.cl:275,329,343)
   // analysis record (sampleAnalyse, opencl/treeletRenderer_rgba.cl:848-875):
   // kind 1 leaf hit, 2 inner node at the LOD / depth limit, 0 none
   uint32_t rec_kind;
@@ -294,6 +295,7 @@ constexpr int kMaxStack = 24;
 inline void leaf_record(const Buffers& B, uint32_t leaf, float scale_exp2, float ratio, float tc_min, Result* r) {
   r->rec_kind = 1;
   r->rec_err = scale_exp2 - (ratio * tc_min);
+  r->quality = r->rec_err;
   r->rec_sub = leaf < B.pool ? B.nodes[leaf].first : 0u;
 }
 inline void stop_record(float scale_exp2, float ratio, float tc_min, Result* r) {
@@ -306,6 +308,7 @@ void sample_treelet(const Buffers& B, const float o[3], float d[3], float ratio,
   r->rec_kind = 0;
   r->rec_err = 0.0f;
   r->rec_sub = 0;
+  r->quality = 0.0f;
   r->hit = false;
   r->loops = 0;
   r->pushes = 0;
@@ -347,6 +350,7 @@ void sample_treelet(const Buffers& B, const float o[3], float d[3], float ratio,
       r->t = parent->parent_tmin;
       r->loops = while_counter;
       std::memcpy(r->center, parent->center, sizeof r->center);
+      r->quality = scale_exp2 - (ratio * parent->parent_tmin);
       return;
     }
     if (near_pop_threshold(parent->parent_tmin, parent->parent_tmax, epsilon)) r->boundary = true;
@@ -375,7 +379,7 @@ void sample_treelet(const Buffers& B, const float o[3], float d[3], float ratio,
         if (near_edge(t0, t1, tc_min, tc_max)) r->boundary = true;
       }
       const uint64_t pidx = parent->parent_node_index;
-      if (pidx >= B.pool) return;  // corrupt index: miss
+      if (pidx >= B.pool) { r->t = t_max; r->loops = while_counter; return; }  // corrupt index: miss
       const Node pn = B.nodes[pidx];
       if (valid_flag(pn.masks, child_idx)) {
         const uint32_t child = parent->parent_node_index + nth_child_idx(pn.masks, pn.first, child_idx);
@@ -423,6 +427,7 @@ void travers_cpu3(const Buffers& B, const float o[3], float d[3], float ratio, R
   r->rec_kind = 0;
   r->rec_err = 0.0f;
   r->rec_sub = 0;
+  r->quality = 0.0f;
   r->hit = false;
   r->loops = 0;
   r->pushes = 0;
@@ -458,6 +463,7 @@ void travers_cpu3(const Buffers& B, const float o[3], float d[3], float ratio, R
     ++while_counter;
     if (while_counter == max_loops) {  // "MAX LOOPS REACHED": no hit
       r->boundary = true;
+      r->loops = while_counter;
       return;
     }
     if (refetch_parent) {
@@ -495,7 +501,7 @@ void travers_cpu3(const Buffers& B, const float o[3], float d[3], float ratio, R
         const float t0[3] = {tx0, ty0, tz0}, t1[3] = {tx1, ty1, tz1};
         if (near_edge(t0, t1, tc_min, tc_max)) r->boundary = true;
       }
-      if (parent_node_index >= B.pool) { r->t = t_max; return; }
+      if (parent_node_index >= B.pool) { r->t = t_max; r->loops = while_counter; return; }
       const Node pn = B.nodes[parent_node_index];
       if (valid_flag(pn.masks, child_index)) {
         const uint32_t child = parent_node_index + nth_child_idx(pn.masks, pn.first, child_index);
@@ -525,6 +531,7 @@ void travers_cpu3(const Buffers& B, const float o[3], float d[3], float ratio, R
       continue;
     }
   }
+  r->loops = while_counter;  // left the root: miss
 }
 
 // Policy B: fillFrameBuffer_rgba.cl::sample (:239-425), the single-SVO
@@ -536,6 +543,7 @@ void sample_fill(const Node* nodes, uint64_t n, const float o[3], float d[3], fl
   r->rec_kind = 0;
   r->rec_err = 0.0f;
   r->rec_sub = 0;
+  r->quality = 0.0f;
   r->hit = false;
   r->loops = 0;
   r->pushes = 0;
@@ -588,11 +596,12 @@ void sample_fill(const Node* nodes, uint64_t n, const float o[3], float d[3], fl
         const float t0[3] = {tx0, ty0, tz0}, t1[3] = {tx1, ty1, tz1};
         if (near_edge(t0, t1, tc_min, tc_max)) r->boundary = true;
       }
-      if (parent->parent_node_index >= n) return;
+      if (parent->parent_node_index >= n) { r->t = t_max; r->loops = while_counter; return; }
       const Node pn = nodes[parent->parent_node_index];
       if (valid_flag(pn.masks, idx)) {
         const uint32_t child = nth_child_idx(pn.masks, pn.first, idx);  // absolute
         if (leaf_flag(pn.masks, idx) || ratio * tc_max > scale_exp2) {
+          if (leaf_flag(pn.masks, idx)) r->quality = scale_exp2 - (ratio * tc_min);
           r->hit = true;
           r->node = child;
           r->depth = scale_max - scale;
@@ -618,6 +627,106 @@ void sample_fill(const Node* nodes, uint64_t n, const float o[3], float d[3], fl
   r->node = 0xFFFFFFFFu;
   r->t = t_max;
   r->loops = while_counter;
+}
+
+// ------------------------------------------------------------ view modes
+// renderToBuffer's modes (treeletRenderer_rgba.cl:644-732), FP32 RGBA,
+// restated: per-component vector arithmetic, dot = (x*x + y*y) + z*z,
+// normalize = v / sqrt(dot(v, v)), OpenCL max(0, x), smoothstep(0, 1, x),
+// the FP64 literals of the phong and colour-map code taken as FP32.
+struct V3 { float x, y, z; };
+inline void unpack_rgba(uint32_t v, float o[4]) {
+  o[0] = (float)((v >> 24) & 255u); o[1] = (float)((v >> 16) & 255u); o[2] = (float)((v >> 8) & 255u); o[3] = (float)(v & 255u);
+}
+inline V3 get_normal(uint32_t packed) {
+  float u[4];
+  unpack_rgba(packed, u);
+  const float k = 0.003921569f;
+  return V3{((u[0] * k) - 0.5f) * 2.0f, ((u[1] * k) - 0.5f) * 2.0f, ((u[2] * k) - 0.5f) * 2.0f};
+}
+inline float dot3(V3 a, V3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+inline V3 normalize3(V3 v) {
+  const float l = std::sqrt(dot3(v, v));
+  return V3{v.x / l, v.y / l, v.z / l};
+}
+inline float smoothstep01(float x) {
+  float t = (x - 0.0f) / (1.0f - 0.0f);
+  t = t < 0.0f ? 0.0f : (t > 1.0f ? 1.0f : t);
+  return t * t * (3.0f - 2.0f * t);
+}
+inline float adjust(float c, float f) {
+  if (c == 0.0f) return 0.0f;
+  int r = (int)std::round(255.0f * std::pow(c * f, 0.8f));
+  return (float)std::min(255, std::max(0, r));
+}
+inline V3 data_point_color(float value, float vmin, float vmax) {
+  const float w = (value - vmin) / (vmax - vmin) * (780.0f - 380.0f) + 380.0f;
+  float r, g, b, f;
+  if (380.0f <= w && w <= 440.0f) { r = -(w - 440.0f) / (440.0f - 380.0f); g = 0.0f; b = 1.0f; }
+  else if (440.0f < w && w <= 490.0f) { r = 0.0f; g = (w - 440.0f) / (490.0f - 440.0f); b = 1.0f; }
+  else if (490.0f < w && w <= 510.0f) { r = 0.0f; g = 1.0f; b = -(w - 510.0f) / (510.0f - 490.0f); }
+  else if (510.0f < w && w <= 580.0f) { r = (w - 510.0f) / (580.0f - 510.0f); g = 1.0f; b = 0.0f; }
+  else if (580.0f < w && w <= 645.0f) { r = 1.0f; g = -(w - 645.0f) / (645.0f - 580.0f); b = 0.0f; }
+  else if (645.0f < w && w <= 780.0f) { r = 1.0f; g = 0.0f; b = 0.0f; }
+  else { r = 0.0f; g = 0.0f; b = 0.0f; }
+  if (380.0f <= w && w <= 420.0f) f = 0.3f + 0.7f * (w - 380.0f) / (420.0f - 380.0f);
+  else if (420.0f < w && w <= 701.0f) f = 1.0f;
+  else if (701.0f < w && w <= 780.0f) f = 0.3f + 0.7f * (780.0f - w) / (780.0f - 701.0f);
+  else f = 0.0f;
+  const float fac = 1.0f / 255.0f;
+  return V3{adjust(r, f) * fac, adjust(g, f) * fac, adjust(b, f) * fac};
+}
+
+void shade(uint32_t mode, const Result& r, bool hit, const float d[3], uint32_t color, uint32_t normal, float out[4]) {
+  out[3] = 1.0f;
+  const float k = 0.003921569f;
+  switch (mode) {
+    case 1:
+      if (hit) { const V3 n = get_normal(normal); out[0] = n.x; out[1] = n.y; out[2] = n.z; }
+      else { out[0] = 0.2f; out[1] = 0.3f; out[2] = 0.2f; }
+      return;
+    case 2: {
+      const V3 c = data_point_color((float)r.loops, 0.0f, (float)(13 * 30));
+      out[0] = c.x; out[1] = c.y; out[2] = c.z;
+      return;
+    }
+    case 3: {
+      const float v = (float)r.depth / (float)13, h = hit ? 1.0f : 0.0f;
+      out[0] = out[1] = out[2] = v * h;
+      return;
+    }
+    case 4:
+      out[0] = out[1] = out[2] = 1.0f - r.t * 0.1f;
+      return;
+    case 5:
+      out[0] = out[1] = out[2] = 1.0f - (float)r.loops / 200.0f;
+      return;
+    case 6: {
+      if (!hit) { out[0] = 0.4f; out[1] = 0.6f; out[2] = 0.4f; return; }
+      const V3 n = normalize3(get_normal(normal));
+      float u[4];
+      unpack_rgba(color, u);
+      const float col[3] = {u[0] * k, u[1] * k, u[2] * k};
+      const V3 l = normalize3(V3{-120.0f - r.center[0], 0.0f - r.center[1], 0.0f - r.center[2]});
+      const float ndl = cl_max(0.0f, dot3(n, l));
+      const V3 rf = normalize3(V3{((2.0f * n.x) * ndl) - l.x, ((2.0f * n.y) * ndl) - l.y, ((2.0f * n.z) * ndl) - l.z});
+      const float rdv = cl_max(0.0f, dot3(rf, V3{-d[0], -d[1], -d[2]}));
+      const float spec = cl_max(0.0f, std::pow(rdv, 10.0f)) * (ndl > 0.0f ? 1.0f : 0.0f);
+      const float shadow = 1.0f - 0.0f;
+      for (int a = 0; a < 3; ++a)
+        out[a] = smoothstep01(col[a] * 0.5f + ((col[a] * ndl) * shadow + (spec * shadow) * 0.2f) * 0.5f);
+      return;
+    }
+    case 7:
+      if (hit) out[0] = out[1] = out[2] = std::sqrt(std::sqrt(r.quality));
+      else { out[0] = 0.6f; out[1] = 0.3f; out[2] = 0.3f; }
+      return;
+    default: {
+      if (hit) { float u[4]; unpack_rgba(color, u); out[0] = u[0] * k; out[1] = u[1] * k; out[2] = u[2] * k; }
+      else { out[0] = 0.2f; out[1] = 0.3f; out[2] = 0.2f; }
+      return;
+    }
+  }
 }
 
 // Single-SVO files, read independently of the library (own parser).
@@ -722,8 +831,10 @@ int oracle_render_buffers(const uint32_t* nodes, uint64_t pool_nodes, const uint
                           uint32_t nodes_per_treelet, uint32_t num_slots, uint32_t depth, const void* camera,
                           int policy, float* t, uint32_t* node, uint32_t* treelet, uint32_t* rgba, uint8_t* boundary,
                           uint32_t* pushes, uint32_t* depth_out, uint32_t* loops, float* center, uint8_t* rec_kind,
-                          float* rec_err, uint32_t* rec_sub, int threads) {
+                          float* rec_err, uint32_t* rec_sub, const uint32_t* normals, float* shade_out,
+                          int threads) {
   if (!nodes || !colors || !slot_gid || !camera || nodes_per_treelet == 0) return 1;
+  if (shade_out && !normals) return 1;
   if (policy != 0 && policy != 2) return 1;
   Camera cam;
   std::memcpy(&cam, camera, sizeof cam);
@@ -735,6 +846,7 @@ int oracle_render_buffers(const uint32_t* nodes, uint64_t pool_nodes, const uint
     for (int x = 0; x < W; ++x) {
       float d[3];
       pixel_ray(cam, (uint32_t)x, (uint32_t)y, d);
+      const float d0[3] = {d[0], d[1], d[2]};  // the traversal clamps d
       const float o[3] = {cam.position[0], cam.position[1], cam.position[2]};
       Result r;
       if (policy == 0) sample_treelet(B, o, d, cam.t_scale_ratio, &r);
@@ -757,6 +869,10 @@ int oracle_render_buffers(const uint32_t* nodes, uint64_t pool_nodes, const uint
       if (rec_kind) rec_kind[p] = (uint8_t)r.rec_kind;
       if (rec_err) rec_err[p] = r.rec_err;
       if (rec_sub) rec_sub[p] = r.rec_sub;
+      if (shade_out) {
+        const bool h = r.hit && r.node < pool_nodes;
+        shade(cam.view_mode, r, h, d0, h ? colors[r.node] : 0u, h ? normals[r.node] : 0u, shade_out + 4 * p);
+      }
     }
   }
   return 0;
@@ -764,11 +880,12 @@ int oracle_render_buffers(const uint32_t* nodes, uint64_t pool_nodes, const uint
 
 int oracle_render(const oracle_scene* s, const void* camera, int policy, float* t, uint32_t* node, uint32_t* treelet,
                   uint32_t* rgba, uint8_t* boundary, uint32_t* pushes, uint32_t* depth_out, uint32_t* loops,
-                  float* center, int threads) {
+                  float* center, float* shade_out, int threads) {
   if (!s || s->incore.empty()) return 6;
   return oracle_render_buffers(reinterpret_cast<const uint32_t*>(s->incore.data()), s->incore.size(), s->colors.data(),
                                s->slot_gid.data(), s->N, s->num_slots, s->max_depth, camera, policy, t, node, treelet,
-                               rgba, boundary, pushes, depth_out, loops, center, nullptr, nullptr, nullptr, threads);
+                               rgba, boundary, pushes, depth_out, loops, center, nullptr, nullptr, nullptr,
+                               s->normals.data(), shade_out, threads);
 }
 
 // One ray with explicit origin / direction (direction used as given).
@@ -816,7 +933,7 @@ void oracle_free_single(oracle_single* s) { delete s; }
 // mode 0): colour = packages[attribIndices[node]] (:640), alpha forced,
 // treelet id 0 for a hit (the whole SVO is one treelet).
 int oracle_render_single(const oracle_single* s, const void* camera, float* t, uint32_t* node, uint32_t* treelet,
-                         uint32_t* rgba, uint8_t* boundary, uint32_t* pushes, int threads) {
+                         uint32_t* rgba, uint8_t* boundary, uint32_t* pushes, float* shade_out, int threads) {
   if (!s || !camera) return 1;
   Camera cam;
   std::memcpy(&cam, camera, sizeof cam);
@@ -826,6 +943,7 @@ int oracle_render_single(const oracle_single* s, const void* camera, float* t, u
     for (int x = 0; x < W; ++x) {
       float d[3];
       pixel_ray(cam, (uint32_t)x, (uint32_t)y, d);
+      const float d0[3] = {d[0], d[1], d[2]};
       const float o[3] = {cam.position[0], cam.position[1], cam.position[2]};
       Result r;
       sample_fill(s->nodes.data(), s->nodes.size(), o, d, cam.t_scale_ratio, &r);
@@ -837,6 +955,10 @@ int oracle_render_single(const oracle_single* s, const void* camera, float* t, u
       if (rgba) rgba[p] = hit ? (s->packages[2 * (size_t)s->aidx[r.node] + 1] | 0xFFu) : 0x334D33FFu;
       if (boundary) boundary[p] = r.boundary ? 1 : 0;
       if (pushes) pushes[p] = r.pushes;
+      if (shade_out) {
+        const size_t a = hit ? 2 * (size_t)s->aidx[r.node] : 0;
+        shade(cam.view_mode, r, hit, d0, hit ? s->packages[a + 1] : 0u, hit ? s->packages[a] : 0u, shade_out + 4 * p);
+      }
     }
   }
   return 0;

@@ -32,7 +32,7 @@ def test_struct_sizes_match_header_layout():
     assert ctypes.sizeof(_capi.svo_node) == 8
     assert ctypes.sizeof(_capi.svo_treelet_header) == 32
     assert ctypes.sizeof(_capi.svo_camera) == 80
-    assert ctypes.sizeof(_capi.svo_frame_out) == 5 * 8
+    assert ctypes.sizeof(_capi.svo_frame_out) == 6 * 8
 
 
 def test_errors_are_status_codes(tmp_path):
