@@ -158,8 +158,7 @@ struct svo_mm {
     free_slots.clear();
     for (uint32_t i = 1; i != num_slots + 1; ++i) free_slots.push_back(num_slots - i);  // top = slot 0
     dirty.clear();
-    search_pos = 1;
-    resident = 0;
+    resident = 0;  // search_pos survives a reset, like the reference's static cursor
   }
 
   // removeTreeletFromIncoreBuffer (src/TreeletMemoryManager.cpp:665-700).
