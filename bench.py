@@ -9,6 +9,8 @@ renders the frame with the fused colour / depth / node / treelet epilogue;
 inputs are resident in HBM before the timed region starts.
 
 Also reported in the same line:
+* ``policies``: the same frame under both GPU traversal policies (the
+  treelet renderer's and the CPU caster's), each with its own tuned shape;
 * ``batched``: 64 views over the depth-11 terrain and a depth-10 sphere
   cloud, resolutions uniform over {256^2, 512^2, 720p, 1080p}, one launch
   captured in a hipGraph, launch shape from the autotuner;
@@ -179,12 +181,14 @@ def ooc_section(policy, frames, depth=13, treelet_bytes=131072, pool_bytes=8 << 
     s = svo.Stream()
     view = [svo.View(mm, cams[0], frame)]
     pages, evictions, upload_bytes, failed = [], [], [], 0
-    t_feedback = t_upload = t_render = 0.0
+    t_feedback = t_analyse = t_upload = t_render = 0.0
     wall0 = time.perf_counter()
     for i, cam in enumerate(cams):
         a = time.perf_counter()
         jit = tuple(float(v) for v in rng.uniform(-1.0, 1.0, 2))
         mm.performAnalysePass(cam, divide, jit, policy, s)
+        s.synchronize()
+        a2 = time.perf_counter()
         st = mm.updateClientSideIncoreBuffer(s)
         b = time.perf_counter()
         up = mm.updateDeviceMemory(s)
@@ -194,6 +198,7 @@ def ooc_section(policy, frames, depth=13, treelet_bytes=131072, pool_bytes=8 << 
         s.synchronize()
         d = time.perf_counter()
         t_feedback += b - a
+        t_analyse += a2 - a
         t_upload += c - b
         t_render += d - c
         pages.append(st["inserted"])
@@ -212,7 +217,9 @@ def ooc_section(policy, frames, depth=13, treelet_bytes=131072, pool_bytes=8 << 
                     f"analyse pass 1/{divide} per axis, starting from the root treelet",
         "frames": frames, "frames_per_s": round(frames / wall, 1),
         "ms_per_frame": round(wall * 1e3 / frames, 3),
-        "ms_feedback": round(t_feedback * 1e3 / frames, 3), "ms_upload": round(t_upload * 1e3 / frames, 3),
+        "ms_feedback": round(t_feedback * 1e3 / frames, 3),
+        "ms_analyse_pass": round(t_analyse * 1e3 / frames, 3),
+        "ms_upload": round(t_upload * 1e3 / frames, 3),
         "ms_render": round(t_render * 1e3 / frames, 3),
         "page_ins_per_frame": round(float(np.mean(pages)), 2), "page_ins_max": int(max(pages)),
         "page_ins_after_frame_30": round(float(np.mean(pages[30:])), 2) if frames > 30 else None,
@@ -222,7 +229,7 @@ def ooc_section(policy, frames, depth=13, treelet_bytes=131072, pool_bytes=8 << 
     }
     log(f"ooc: {res['frames_per_s']} frames/s, {res['page_ins_per_frame']} page-ins/frame, "
         f"{res['upload_mib_per_frame']} MiB/frame uploaded, render {res['ms_render']} ms, "
-        f"feedback {res['ms_feedback']} ms, upload {res['ms_upload']} ms")
+        f"feedback {res['ms_feedback']} ms (analyse pass {res['ms_analyse_pass']} ms), upload {res['ms_upload']} ms")
     return res
 
 
@@ -341,6 +348,15 @@ def main():
     log(f"{args.steps} launches: {ms:.4f} ms/launch (events), wall {wall * 1e3 / args.steps:.4f} ms/launch, "
         f"{value:.1f} Mrays/s, {achieved:.1f} GB/s algorithmic")
 
+    # the other traversal policy on the same frame (autotuned on its own)
+    other = "cpu3" if args.policy == "treelet" else "treelet"
+    oshape, _ = renderer.autotune(view, other, stream, iters=3)
+    ms_other = renderer.timed(view, other, stream, iters=max(10, args.steps // 2))
+    log(f"{other} policy: {oshape} {ms_other:.4f} ms/launch = {rays / ms_other / 1e3:.1f} Mrays/s")
+    policies = {args.policy: {"mrays_per_s": round(value, 1), "ms_per_launch": round(ms, 5), "launch_shape": list(shape)},
+                other: {"mrays_per_s": round(rays / ms_other / 1e3, 1), "ms_per_launch": round(ms_other, 5),
+                        "launch_shape": list(oshape)}}
+
     if args.copy_calibration_mb > 0:
         nbytes = args.copy_calibration_mb << 20
         src, dst = svo.DeviceBuffer(nbytes), svo.DeviceBuffer(nbytes)
@@ -373,6 +389,7 @@ def main():
             "mean_pushes": round(float(res["pushes"].mean()), 3),
             "launch_shape": list(shape),
         },
+        "policies": policies,
         "roofline": {
             "bound": "hbm",
             "achieved": round(achieved, 2),

@@ -27,8 +27,20 @@
 // direction instead of compare-and-swap, two planes per axis per child step,
 // the parent's tmax restored from the stack on a pop, cycle checks scheduled
 // on the wave's (scalar) trip count, and a single loop exit (exits inside the
-// loop make the compiler copy the loop state at every exit).  A wave with any
-// other ray runs the generic loop.
+// loop make the compiler copy the loop state at every exit).  One trip of
+// the fast loop is up to two of the reference's iterations: a pop, if the
+// ray pops, then a child step, if the ray does not pop again.  In a wave
+// some lanes pop while others step; in one trip, a lane that pops also takes
+// its next step instead of idling through the others' step code.  The
+// iteration count (loops, maxLoops) still counts every pop and every step.
+// A wave with any other ray runs the generic loop.
+//
+// Tile rows are dispatched bottom-up.  The last waves of a launch run almost
+// alone on their SIMDs, where node-fetch latency, not VALU issue, sets their
+// pace; in upright views the rows below the horizon hold the long traversals
+// and the rows above it the short ones, so bottom-up ends a launch on short
+// traversals (1080p terrain frame: 0.350 -> 0.291 ms).  The order changes
+// no pixel.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -431,8 +443,6 @@ __device__ Hit traverse_fast(const Uniform& U, const float o[3], const float d[3
   float hit_err = 0.0f;
 
   do {
-    const float se = exp2i(scale - U.smax);  // half-edge of the current node
-    ++iter;
     ++trip;
     // snapshots at power-of-two trips >= 128, comparisons every 8th trip: a
     // repeat found at any multiple of the period is a period too
@@ -442,92 +452,97 @@ __device__ Hit traverse_fast(const Uniform& U, const float o[3], const float d[3
         snap_tmin = __float_as_uint(tmin); snap_tmax = __float_as_uint(tmax); snap_pushes = pushes;
       } else if (snap_iter != 0u && (uint32_t)scale == snap_scale && path == snap_path &&
                  __float_as_uint(tmin) == snap_tmin && __float_as_uint(tmax) == snap_tmax &&
-                 iter < U.max_loops) {
+                 iter + 2u < U.max_loops) {
         const uint32_t p = iter - snap_iter;
-        const uint32_t skip = (U.max_loops - iter) / p;
+        const uint32_t skip = (U.max_loops - 2u - iter) / p;
         pushes += skip * (pushes - snap_pushes);
         iter += skip * p;
         snap_iter = 0u;
       }
     }
-    if (iter == U.max_loops) {
-      outcome = 3;
-      break;
-    }
-    const bool pop = policy_pop<POLICY>(tmin, tmax);
-
-    bool fetch = false;
-    uint32_t fidx = 0;
-    if (!pop) {
-      const float te = tmin + eps;
-      bool up[3];
-      up[0] = ((o[0] + te * d[0]) - c[0]) > 0.0f;
-      up[1] = ((o[1] + te * d[1]) - c[1]) > 0.0f;
-      up[2] = ((o[2] + te * d[2]) - c[2]) > 0.0f;
-      const uint32_t ci = (up[0] ? 4u : 0u) | (up[1] ? 2u : 0u) | (up[2] ? 1u : 0u);
-      float t0[3], t1[3];
-#pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        const float tmid = rc[a] * c[a] + mo[a];
-        const float tface = rc[a] * (c[a] + (up[a] ? se : -se)) + mo[a];
-        const bool nearside = up[a] == neg[a];  // child on the near side of the midplane
-        t0[a] = nearside ? tface : tmid;
-        t1[a] = nearside ? tmid : tface;
-      }
-      const float tc_min = cl_max(t0[0], cl_max(t0[1], t0[2]));
-      const float tc_max = cl_min(t1[0], cl_min(t1[1], t1[2]));
-      const uint32_t masks = (uint32_t)(nd >> 32);
-      if ((masks >> (8u + ci)) & 1u) {
-        const uint32_t child = nidx + (uint32_t)nd + (uint32_t)__popc((masks >> 8) & ((1u << ci) - 1u));  // relative
-        bool stop;
-        if (POLICY == 0) stop = (ratio * tc_min > se) || scale == U.lod_scale;
-        else if (POLICY == 1) stop = ratio * tc_max > se;
-        else stop = (ratio * tc_max > se) || scale == 0;
-        if (((masks >> ci) & 1u) || stop) {  // leaf, or inner node at the LOD / depth limit
-          outcome = 1;
-          hit_node = child;
-          const bool is_leaf = (masks >> ci) & 1u;
-          hit_kind = is_leaf ? 1u : 2u;
-          hit_ci = ci;
-          hit_err = is_leaf ? leaf_error(se, ratio, tc_min) : stop_error(se, ratio, tc_min);
-        } else if (child >= U.pool_nodes) {
-          outcome = 4;
-        } else {
-          ++pushes;
-          stk.put2(scale - U.mps, nidx, tmax);
-          path = (path << 3) | ci;
-          --scale;
-          const float h = se * 0.5f;
-#pragma unroll
-          for (int a = 0; a < 3; ++a) c[a] = c[a] + (up[a] ? h : -h);
-          tmin = tc_min;
-          tmax = tc_max;
-          fidx = child;
-          fetch = true;
-        }
+    // first half of a trip: one reference iteration if it pops
+    if (policy_pop<POLICY>(tmin, tmax)) {
+      ++iter;
+      if (iter >= U.max_loops) {
+        outcome = 3;
+      } else if (scale + 1 >= U.smax) {
+        outcome = 2;  // left the root
       } else {
-        tmin = tc_max;  // advance
+        const float se = exp2i(scale - U.smax);
+        ++scale;
+        float tparent;
+        uint32_t fidx;
+        stk.get2(scale - U.mps, fidx, tparent);
+        const uint32_t pl = (uint32_t)path;
+        path >>= 3;
+        const uint32_t seb = __float_as_uint(se);
+        // parent centre = child centre -/+ se (bit set: the child was the upper half)
+        c[0] = c[0] + __uint_as_float(seb | ((pl << 29) & 0x80000000u));
+        c[1] = c[1] + __uint_as_float(seb | ((pl << 30) & 0x80000000u));
+        c[2] = c[2] + __uint_as_float(seb | (pl << 31));
+        tmin = tmax;
+        tmax = tparent;
+        nidx = fidx;
+        nd = U.nodes[fidx];
       }
-    } else if (scale + 1 >= U.smax) {
-      outcome = 2;  // left the root
-    } else {
-      ++scale;
-      float tparent;
-      stk.get2(scale - U.mps, fidx, tparent);
-      fetch = true;
-      const uint32_t pl = (uint32_t)path;
-      path >>= 3;
-      const uint32_t seb = __float_as_uint(se);
-      // parent centre = child centre -/+ se (bit set: the child was the upper half)
-      c[0] = c[0] + __uint_as_float(seb | ((pl << 29) & 0x80000000u));
-      c[1] = c[1] + __uint_as_float(seb | ((pl << 30) & 0x80000000u));
-      c[2] = c[2] + __uint_as_float(seb | (pl << 31));
-      tmin = tmax;
-      tmax = tparent;
     }
-    if (fetch) {
-      nidx = fidx;
-      nd = U.nodes[fidx];
+    // second half: the next reference iteration if it steps into a child
+    if (outcome == 0 && !policy_pop<POLICY>(tmin, tmax)) {
+      ++iter;
+      if (iter >= U.max_loops) {
+        outcome = 3;
+      } else {
+        const float se = exp2i(scale - U.smax);  // half-edge of the current node
+        const float te = tmin + eps;
+        bool up[3];
+        up[0] = ((o[0] + te * d[0]) - c[0]) > 0.0f;
+        up[1] = ((o[1] + te * d[1]) - c[1]) > 0.0f;
+        up[2] = ((o[2] + te * d[2]) - c[2]) > 0.0f;
+        const uint32_t ci = (up[0] ? 4u : 0u) | (up[1] ? 2u : 0u) | (up[2] ? 1u : 0u);
+        float t0[3], t1[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          const float tmid = rc[a] * c[a] + mo[a];
+          const float tface = rc[a] * (c[a] + (up[a] ? se : -se)) + mo[a];
+          const bool nearside = up[a] == neg[a];  // child on the near side of the midplane
+          t0[a] = nearside ? tface : tmid;
+          t1[a] = nearside ? tmid : tface;
+        }
+        const float tc_min = cl_max(t0[0], cl_max(t0[1], t0[2]));
+        const float tc_max = cl_min(t1[0], cl_min(t1[1], t1[2]));
+        const uint32_t masks = (uint32_t)(nd >> 32);
+        if ((masks >> (8u + ci)) & 1u) {
+          const uint32_t child = nidx + (uint32_t)nd + (uint32_t)__popc((masks >> 8) & ((1u << ci) - 1u));  // relative
+          bool stop;
+          if (POLICY == 0) stop = (ratio * tc_min > se) || scale == U.lod_scale;
+          else if (POLICY == 1) stop = ratio * tc_max > se;
+          else stop = (ratio * tc_max > se) || scale == 0;
+          if (((masks >> ci) & 1u) || stop) {  // leaf, or inner node at the LOD / depth limit
+            outcome = 1;
+            hit_node = child;
+            const bool is_leaf = (masks >> ci) & 1u;
+            hit_kind = is_leaf ? 1u : 2u;
+            hit_ci = ci;
+            hit_err = is_leaf ? leaf_error(se, ratio, tc_min) : stop_error(se, ratio, tc_min);
+          } else if (child >= U.pool_nodes) {
+            outcome = 4;
+          } else {
+            ++pushes;
+            stk.put2(scale - U.mps, nidx, tmax);
+            path = (path << 3) | ci;
+            --scale;
+            const float h = se * 0.5f;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) c[a] = c[a] + (up[a] ? h : -h);
+            tmin = tc_min;
+            tmax = tc_max;
+            nidx = child;
+            nd = U.nodes[child];
+          }
+        } else {
+          tmin = tc_max;  // advance
+        }
+      }
     }
   } while (outcome == 0);
 
@@ -739,8 +754,9 @@ __global__ __launch_bounds__(64 * WAVES) void trace_kernel(const TraceFrame* __r
   }
   const TraceFrame& F = frames[lo];
   const uint32_t lt = tile - F.tile_begin;
+  const uint32_t tiles_y = (F.cam.height + TH - 1u) / TH;
   const uint32_t x = (lt % F.tiles_x) * TW + (lane % TW);
-  const uint32_t y = (lt / F.tiles_x) * TH + (lane / TW);
+  const uint32_t y = (tiles_y - 1u - lt / F.tiles_x) * TH + (lane / TW);
   if (x >= F.cam.width || y >= F.cam.height) return;
   const TracePools& S = F.pools;
 
