@@ -21,6 +21,7 @@
 #include <memory>
 #include <queue>
 #include <set>
+#include <unordered_map>
 #include <vector>
 
 #include "svo_internal.h"
@@ -119,6 +120,12 @@ struct svo_mm {
   uint32_t* h_fb_count = nullptr;
   uint32_t fb_stamp = 1;
   uint64_t evictions = 0;  // treelets freed by freeIncorePosition (incl. descendants)
+  // uploads: span descriptors (pinned, and their device copy read by the
+  // upload kernel) and the pinned slot table
+  UploadSpan* h_spans = nullptr;
+  UploadSpan* d_spans = nullptr;
+  uint32_t span_cap = 0;
+  int32_t* h_slot_tab = nullptr;
 
   bool host_only() const { return device < 0; }
 
@@ -142,6 +149,9 @@ struct svo_mm {
     if (h_fb_out) hipHostFree(h_fb_out);
     if (h_fb_queue) hipHostFree(h_fb_queue);
     if (h_fb_count) hipHostFree(h_fb_count);
+    hipFree(d_spans);
+    if (h_spans) hipHostFree(h_spans);
+    if (h_slot_tab) hipHostFree(h_slot_tab);
     hipSetDevice(cur);
   }
 
@@ -166,10 +176,7 @@ struct svo_mm {
     const svo_treelet_header& h = scene->treelets[gid].h;
     const int32_t slot = treelet_slot[gid];
     const int32_t pslot = treelet_slot[h.parent_gid];
-    const uint64_t leaf = (uint64_t)pslot * N + h.parent_leaf_pos;
-    const uint64_t leaf_parent = (uint64_t)pslot * N + h.parent_leaf_parent_pos;
-    incore[leaf] = scene->treelets[h.parent_gid].nodes[h.parent_leaf_pos];  // pristine leaf
-    incore[leaf_parent].masks |= 1u << (kLeafShift + h.parent_leaf_idx);
+    data_op(DataOp{DataOp::kUnstitch, (uint32_t)pslot, gid, 0});
     mark((uint32_t)pslot);
     slots[slot] = SlotInfo();
     free_slots.push_back((uint32_t)slot);
@@ -273,10 +280,12 @@ struct svo_mm {
       if (sl >= 0) slots[sl].visibility = kMaxVisibility;
     }
     const uint64_t ev0 = evictions;
+    deferring = true;
     for (const auto& e : asked) {
       if (insert(e.first, e.second) == SVO_OK) ++S.inserted;
       else ++S.insert_failed;
     }
+    replay_oplog();
     clear_error();
     S.evicted = (uint32_t)(evictions - ev0);
     S.resident_treelets = resident;
@@ -303,46 +312,111 @@ struct svo_mm {
     slots[slot] = SlotInfo{(int32_t)gid, kMaxVisibility, error};
     treelet_slot[gid] = (int32_t)slot;
     ++resident;
-    const uint64_t pos = (uint64_t)slot * N;
-    const TreeletData& T = scene->treelets[gid];
-    std::copy(T.nodes.begin(), T.nodes.end(), incore.begin() + pos);
-    std::fill(incore.begin() + pos + T.h.num_nodes, incore.begin() + pos + N, svo_node{0, 0});
-    for (uint32_t i = 0; i < T.h.num_nodes; ++i) {
-      normals[pos + i] = T.attribs[2 * (size_t)i];
-      colors[pos + i] = T.attribs[2 * (size_t)i + 1];
-    }
-    std::fill(normals.begin() + pos + T.h.num_nodes, normals.begin() + pos + N, 0u);
-    std::fill(colors.begin() + pos + T.h.num_nodes, colors.begin() + pos + N, 0u);
+    data_op(DataOp{DataOp::kFill, slot, gid, 0});
     mark(slot);
     if (gid == 0) return SVO_OK;
-    // stitch: the child root replaces the parent's leaf, which stops being a
-    // leaf and points (relatively, modulo 2^32) at the child's level 1.
     const int32_t pslot = treelet_slot[h.parent_gid];
-    const uint64_t leaf = (uint64_t)pslot * N + h.parent_leaf_pos;
-    const uint64_t leaf_parent = (uint64_t)pslot * N + h.parent_leaf_parent_pos;
-    incore[leaf] = incore[pos];
-    incore[leaf_parent].masks &= ~(1u << (kLeafShift + h.parent_leaf_idx));
-    incore[leaf].first_child = (uint32_t)(pos + 1) - (uint32_t)leaf;
+    data_op(DataOp{DataOp::kStitch, (uint32_t)pslot, gid, slot});
     mark((uint32_t)pslot);
     children_resident[h.parent_gid].insert(gid);
     return SVO_OK;
   }
 
-  svo_status upload_range(uint32_t first_slot, uint32_t count, hipStream_t st, uint64_t* bytes) {
-    const size_t n0 = (size_t)first_slot * N, n = (size_t)count * N;
-    if (bytes) *bytes += n * 16;
-    if (host_only()) return SVO_OK;
-    HIP_TRY(hipMemcpyAsync(d_nodes + n0, incore.data() + n0, n * 8, hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(d_colors + n0, colors.data() + n0, n * 4, hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(d_normals + n0, normals.data() + n0, n * 4, hipMemcpyHostToDevice, st));
-    return SVO_OK;
+  // ---- data movement of inserts and removes.  Outside apply_records each
+  // operation runs at once; inside, they are logged and replayed at its end:
+  // the last fill of every slot in parallel (a fill rewrites the whole slot),
+  // then, in log order, every stitch/unstitch that no later fill of its slot
+  // overwrites.  Each slot ends as the operations in log order leave it.
+  struct DataOp {
+    enum : uint32_t { kFill, kStitch, kUnstitch };
+    uint32_t kind;
+    uint32_t slot;       // the slot written (for a stitch/unstitch: the parent's)
+    uint32_t gid;        // the treelet filled, stitched or unstitched
+    uint32_t src_slot;   // stitch: the slot holding the child
+  };
+  bool deferring = false;
+  std::vector<DataOp> oplog;
+
+  void data_op(const DataOp& op) {
+    if (deferring) oplog.push_back(op);
+    else run_op(op);
   }
 
-  svo_status upload_slot_table(hipStream_t st) {
+  void run_op(const DataOp& op) {
+    const svo_treelet_header& h = scene->treelets[op.gid].h;
+    if (op.kind == DataOp::kFill) {
+      const uint64_t pos = (uint64_t)op.slot * N;
+      const TreeletData& T = scene->treelets[op.gid];
+      std::copy(T.nodes.begin(), T.nodes.end(), incore.begin() + pos);
+      std::fill(incore.begin() + pos + T.h.num_nodes, incore.begin() + pos + N, svo_node{0, 0});
+      for (uint32_t i = 0; i < T.h.num_nodes; ++i) {
+        normals[pos + i] = T.attribs[2 * (size_t)i];
+        colors[pos + i] = T.attribs[2 * (size_t)i + 1];
+      }
+      std::fill(normals.begin() + pos + T.h.num_nodes, normals.begin() + pos + N, 0u);
+      std::fill(colors.begin() + pos + T.h.num_nodes, colors.begin() + pos + N, 0u);
+      return;
+    }
+    const uint64_t leaf = (uint64_t)op.slot * N + h.parent_leaf_pos;
+    const uint64_t leaf_parent = (uint64_t)op.slot * N + h.parent_leaf_parent_pos;
+    if (op.kind == DataOp::kStitch) {
+      // the child root replaces the parent's leaf, which stops being a leaf
+      // and points (relatively, modulo 2^32) at the child's level 1; a child
+      // is stitched right after its own fill, so its root is the pristine one
+      const uint64_t pos = (uint64_t)op.src_slot * N;
+      incore[leaf] = scene->treelets[op.gid].nodes[0];
+      incore[leaf_parent].masks &= ~(1u << (kLeafShift + h.parent_leaf_idx));
+      incore[leaf].first_child = (uint32_t)(pos + 1) - (uint32_t)leaf;
+    } else {
+      incore[leaf] = scene->treelets[h.parent_gid].nodes[h.parent_leaf_pos];  // pristine leaf
+      incore[leaf_parent].masks |= 1u << (kLeafShift + h.parent_leaf_idx);
+    }
+  }
+
+  void replay_oplog() {
+    deferring = false;
+    std::unordered_map<uint32_t, size_t> last_fill;
+    for (size_t i = 0; i < oplog.size(); ++i)
+      if (oplog[i].kind == DataOp::kFill) last_fill[oplog[i].slot] = i;
+    std::vector<size_t> fills;
+    fills.reserve(last_fill.size());
+    for (const auto& e : last_fill) fills.push_back(e.second);
+#pragma omp parallel for schedule(dynamic, 1) if (fills.size() > 1)
+    for (size_t k = 0; k < fills.size(); ++k) run_op(oplog[fills[k]]);
+    for (size_t i = 0; i < oplog.size(); ++i) {
+      if (oplog[i].kind == DataOp::kFill) continue;
+      const auto it = last_fill.find(oplog[i].slot);
+      if (it != last_fill.end() && it->second > i) continue;  // overwritten by a later fill
+      run_op(oplog[i]);
+    }
+    oplog.clear();
+  }
+
+  // Uploads runs of slots {first, count} (nodes, colours, normals) and the
+  // slot table in one upload-kernel launch reading the pinned host buffers,
+  // then waits for it: the host buffers may change again once this returns.
+  svo_status upload_runs(const std::vector<std::pair<uint32_t, uint32_t>>& runs, hipStream_t st, uint64_t* bytes) {
+    for (const auto& r : runs)
+      if (bytes) *bytes += (uint64_t)r.second * N * 16;
     if (host_only()) return SVO_OK;
-    std::vector<int32_t> tab(num_slots);
-    for (uint32_t s = 0; s < num_slots; ++s) tab[s] = slots[s].gid;
-    HIP_TRY(hipMemcpyAsync(d_slot_gid, tab.data(), (size_t)num_slots * 4, hipMemcpyHostToDevice, st));
+    uint32_t n = 0;
+    uint64_t chunks = 0;
+    auto add = [&](const void* src, void* dst, uint64_t b) {
+      h_spans[n++] = UploadSpan{(uint64_t)(uintptr_t)src, (uint64_t)(uintptr_t)dst, b, chunks};
+      chunks += (b + 15) / 16;
+    };
+    if (runs.size() * 3 + 1 > span_cap) { set_error("upload: %zu runs exceed the span table", runs.size()); return SVO_ERR_STATE; }
+    for (const auto& r : runs) {
+      const size_t n0 = (size_t)r.first * N, nn = (size_t)r.second * N;
+      add(incore.data() + n0, d_nodes + n0, nn * 8);
+      add(colors.data() + n0, d_colors + n0, nn * 4);
+      add(normals.data() + n0, d_normals + n0, nn * 4);
+    }
+    for (uint32_t s = 0; s < num_slots; ++s) h_slot_tab[s] = slots[s].gid;
+    add(h_slot_tab, d_slot_gid, (uint64_t)num_slots * 4);
+    HIP_TRY(hipMemcpyAsync(d_spans, h_spans, (size_t)n * sizeof(UploadSpan), hipMemcpyHostToDevice, st));
+    const int e = launch_upload(d_spans, n, chunks, st);
+    if (e != 0) { set_error("upload launch failed: %s", hipGetErrorString((hipError_t)e)); return SVO_ERR_HIP; }
     HIP_TRY(hipStreamSynchronize(st));
     return SVO_OK;
   }
@@ -354,6 +428,7 @@ struct svo_mm {
     s.slots_uploaded = (uint32_t)dirty.size();
     if (!dirty.empty()) {
       const float limit = 0.6f * 2.0f;
+      std::vector<std::pair<uint32_t, uint32_t>> runs;
       auto it = dirty.begin();
       while (it != dirty.end()) {
         const uint32_t start = *it;
@@ -370,10 +445,9 @@ struct svo_mm {
           }
         }
         ++s.copy_calls;
-        svo_status r = upload_range(start, num_slots_r, st, &s.bytes);
-        if (r != SVO_OK) return r;
+        runs.push_back({start, num_slots_r});
       }
-      svo_status r = upload_slot_table(st);
+      svo_status r = upload_runs(runs, st, &s.bytes);
       if (r != SVO_OK) return r;
       dirty.clear();
     }
@@ -440,10 +514,15 @@ extern "C" svo_status svo_mm_create(const svo_scene* scene, uint64_t incore_byte
   HIP_TRY(hipMemset(mm->d_fb_slot, 0, slots * 8));
   HIP_TRY(hipMemset(mm->d_fb_gid, 0, ng * 8));
   HIP_TRY(hipMemset(mm->d_fb_count, 0, 4));
+  mm->span_cap = (uint32_t)(3 * slots + 1);
+  if (hipHostMalloc(&mm->h_spans, (size_t)mm->span_cap * sizeof(UploadSpan), 0) != hipSuccess ||
+      hipMalloc(&mm->d_spans, (size_t)mm->span_cap * sizeof(UploadSpan)) != hipSuccess ||
+      hipHostMalloc(&mm->h_slot_tab, slots * 4, 0) != hipSuccess) {
+    set_error("svo_mm_create: upload tables for %llu slots could not be allocated", (unsigned long long)slots);
+    return SVO_ERR_NOMEM;
+  }
   // initClBuffer: the whole incore buffer goes up once
-  r = mm->upload_range(0, mm->num_slots, nullptr, nullptr);
-  if (r != SVO_OK) return r;
-  r = mm->upload_slot_table(nullptr);
+  r = mm->upload_runs({{0u, mm->num_slots}}, nullptr, nullptr);
   if (r != SVO_OK) return r;
   mm->dirty.clear();
   *out = mm.release();

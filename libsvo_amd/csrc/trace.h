@@ -93,4 +93,21 @@ int launch_ray_dirs(const svo_camera& cam, float* dirs_dev, void* stream);
 // Copies bytes/16 16-byte words with a plain streaming kernel.
 int launch_copy(void* dst, const void* src, size_t bytes, void* stream);
 
+// One span of an upload: `bytes` (a multiple of 4) from `src` (pinned host
+// memory or device memory) to `dst` (device memory); first_chunk is the
+// running sum of ceil(bytes / 16) over the spans before it.
+struct UploadSpan {
+  uint64_t src;
+  uint64_t dst;
+  uint64_t bytes;
+  uint64_t first_chunk;
+};
+
+// Copies every span of `spans` (device memory, ascending first_chunk,
+// total_chunks 16-byte chunks in all) in one launch.  Reading pinned host
+// memory from a kernel streams over PCIe at the rate of one large DMA copy
+// whatever the number of spans, where one DMA command per span pays a
+// fixed cost per command.
+int launch_upload(const UploadSpan* spans, uint32_t num_spans, uint64_t total_chunks, void* stream);
+
 }  // namespace svo_impl

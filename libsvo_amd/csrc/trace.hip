@@ -872,6 +872,33 @@ __global__ __launch_bounds__(256) void copy_kernel(uint4* __restrict__ dst, cons
     dst[i] = src[i];
 }
 
+// One lane per 16-byte chunk of the concatenated spans; the span of a chunk
+// is found by binary search over first_chunk.  A chunk moves as one 16-byte
+// word when source and destination are 16-byte aligned, else as 4-byte words.
+__global__ __launch_bounds__(256) void upload_kernel(const UploadSpan* __restrict__ spans, uint32_t num_spans,
+                                                     uint64_t total_chunks) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < total_chunks; c += stride) {
+    uint32_t lo = 0, hi = num_spans;
+    while (hi - lo > 1u) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (spans[mid].first_chunk <= c) lo = mid;
+      else hi = mid;
+    }
+    const UploadSpan sp = spans[lo];
+    const uint64_t off = (c - sp.first_chunk) * 16u;
+    if (off >= sp.bytes) continue;
+    const uint64_t n = sp.bytes - off < 16u ? sp.bytes - off : 16u;
+    const uint64_t s = sp.src + off, d = sp.dst + off;
+    if (n == 16u && ((s | d) & 15u) == 0u) {
+      *reinterpret_cast<uint4*>(d) = *reinterpret_cast<const uint4*>(s);
+    } else {
+      for (uint64_t k = 0; k + 4u <= n; k += 4u)
+        *reinterpret_cast<uint32_t*>(d + k) = *reinterpret_cast<const uint32_t*>(s + k);
+    }
+  }
+}
+
 using TraceFn = void (*)(const TraceFrame*, uint32_t, uint32_t);
 
 template <int POLICY, int TW, int WAVES>
@@ -910,6 +937,13 @@ int launch_copy(void* dst, const void* src, size_t bytes, void* stream) {
   if (n == 0) return 0;
   const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 256 * 64);
   hipLaunchKernelGGL(copy_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream, (uint4*)dst, (const uint4*)src, n);
+  return (int)hipGetLastError();
+}
+
+int launch_upload(const UploadSpan* spans, uint32_t num_spans, uint64_t total_chunks, void* stream) {
+  if (num_spans == 0 || total_chunks == 0) return 0;
+  const unsigned blocks = (unsigned)std::min<uint64_t>((total_chunks + 255) / 256, 2048);
+  hipLaunchKernelGGL(upload_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream, spans, num_spans, total_chunks);
   return (int)hipGetLastError();
 }
 
