@@ -148,11 +148,37 @@ class RenderPassAnalyseHip {
     check(svo_mm_analyse(_mm->handle(), &camera, frameBufferToFeedbackBufferRatio, jx, jy, policy, stream),
           "performAnalysePass");
   }
+  // getVisibleOldTreeletsGids / getVisibleNewTreeletsGids: the visible sets
+  // of the records since the last update (svo_mm_read_feedback); the next
+  // updateClientSideIncoreBuffer applies these records.
+  struct TreeletGidAndError {
+    unsigned _treeletGid;
+    float _error;
+  };
+  std::vector<TreeletGidAndError> getVisibleOldTreeletsGids(void* stream = nullptr) { return visible(stream, true); }
+  std::vector<TreeletGidAndError> getVisibleNewTreeletsGids(void* stream = nullptr) { return visible(stream, false); }
+
   void setEnableRumble(bool onOrOff) { _rumble = onOrOff; }
   bool getEnableRumble() const { return _rumble; }
   TreeletMemoryManagerHip* memoryManager() { return _mm; }
 
  private:
+  std::vector<TreeletGidAndError> visible(void* stream, bool old) {
+    uint32_t n_old = 0, n_new = 0;
+    check(svo_mm_read_feedback(_mm->handle(), stream, nullptr, nullptr, 0, &n_old, nullptr, nullptr, 0, &n_new),
+          "getVisibleTreeletsGids");
+    std::vector<uint32_t> og(n_old), ng(n_new);
+    std::vector<float> oe(n_old), ne(n_new);
+    check(svo_mm_read_feedback(_mm->handle(), stream, og.data(), oe.data(), n_old, &n_old, ng.data(), ne.data(), n_new,
+                               &n_new),
+          "getVisibleTreeletsGids");
+    std::vector<TreeletGidAndError> out;
+    const std::vector<uint32_t>& g = old ? og : ng;
+    const std::vector<float>& e = old ? oe : ne;
+    for (size_t i = 0; i < g.size(); ++i) out.push_back(TreeletGidAndError{g[i], e[i]});
+    return out;
+  }
+
   TreeletMemoryManagerHip* _mm;
   unsigned _w, _h;
   bool _rumble = true;

@@ -465,6 +465,25 @@ typedef struct svo_feedback_stats {
 svo_status svo_mm_apply_feedback(svo_mm* mm, void* stream,
                                  svo_feedback_stats* stats);
 
+/* The visible sets of the records gathered since the previous update, as
+ * RenderPassAnalyse::getVisibleOldTreeletsGids / getVisibleNewTreeletsGids
+ * (src/RenderPassAnalyse.h:89-93) give them after performAnalysePass
+ * (src/RenderPassAnalyse.cpp:248-356):
+ *   old: every treelet holding a recorded hit and all its ancestors, with
+ *        the largest error below it (the root: 99), in ascending gid;
+ *   new: the treelets asked for, by descending error (ties: ascending gid),
+ *        capped at 12 MiB of treelets.
+ * Reads the tables back on `stream` and ends the recording cycle (later
+ * recordings start a new one); the records read are kept, and the next
+ * svo_mm_apply_feedback applies them instead of reading the tables.  Up to
+ * old_cap / new_cap entries are written; *n_old / *n_new receive the full
+ * counts (arrays may be null when their cap is 0). */
+svo_status svo_mm_read_feedback(svo_mm* mm, void* stream,
+                                uint32_t* old_gids, float* old_errors,
+                                uint32_t old_cap, uint32_t* n_old,
+                                uint32_t* new_gids, float* new_errors,
+                                uint32_t new_cap, uint32_t* n_new);
+
 /* The same update from caller-supplied aggregated records (no device
  * access; works on host-only handles): slots[i] was hit with largest error
  * slot_errors[i]; gids[i] (distinct, non-root) asked for with largest error

@@ -17,7 +17,7 @@ from __future__ import annotations
 
 import ctypes
 from dataclasses import dataclass
-from typing import Dict, List, Optional, Sequence
+from typing import Dict, List, Optional, Sequence, Tuple
 
 import numpy as np
 
@@ -283,6 +283,32 @@ class TreeletMemoryManagerHip:
         check(lib.svo_mm_apply_feedback(self._h, stream.handle if stream else None, ctypes.byref(st)),
               "svo_mm_apply_feedback")
         return {k: getattr(st, k) for k, _ in _capi.svo_feedback_stats._fields_}
+
+    def readFeedback(self, stream: Optional["Stream"] = None) -> Dict[str, List[Tuple[int, float]]]:
+        """The visible sets of the records gathered since the last update
+        (RenderPassAnalyse::getVisibleOldTreeletsGids / getVisibleNewTreeletsGids):
+        ``old`` (gid, error) in ascending gid, ``new`` (gid, error) by
+        descending error, capped.  The next updateClientSideIncoreBuffer
+        applies these records."""
+        sh = stream.handle if stream else None
+        n_old, n_new = ctypes.c_uint32(), ctypes.c_uint32()
+        check(lib.svo_mm_read_feedback(self._h, sh, None, None, 0, ctypes.byref(n_old), None, None, 0,
+                                       ctypes.byref(n_new)), "svo_mm_read_feedback")
+        og = np.zeros(max(1, n_old.value), np.uint32)
+        oe = np.zeros(max(1, n_old.value), np.float32)
+        ng = np.zeros(max(1, n_new.value), np.uint32)
+        ne = np.zeros(max(1, n_new.value), np.float32)
+        check(lib.svo_mm_read_feedback(self._h, sh, og.ctypes.data, oe.ctypes.data, og.size, ctypes.byref(n_old),
+                                       ng.ctypes.data, ne.ctypes.data, ng.size, ctypes.byref(n_new)),
+              "svo_mm_read_feedback")
+        return {"old": [(int(g), float(e)) for g, e in zip(og[:n_old.value], oe[:n_old.value])],
+                "new": [(int(g), float(e)) for g, e in zip(ng[:n_new.value], ne[:n_new.value])]}
+
+    def getVisibleOldTreeletsGids(self, stream: Optional["Stream"] = None) -> List[Tuple[int, float]]:
+        return self.readFeedback(stream)["old"]
+
+    def getVisibleNewTreeletsGids(self, stream: Optional["Stream"] = None) -> List[Tuple[int, float]]:
+        return self.readFeedback(stream)["new"]
 
     def applyRecords(self, slots, slot_errors, gids, gid_errors) -> Dict[str, int]:
         """The same update from aggregated records given by the caller."""

@@ -126,6 +126,9 @@ struct svo_mm {
   UploadSpan* d_spans = nullptr;
   uint32_t span_cap = 0;
   int32_t* h_slot_tab = nullptr;
+  // records read by svo_mm_read_feedback, applied by the next apply
+  bool fb_cached = false;
+  std::vector<std::pair<uint32_t, float>> fb_seen, fb_asked;
 
   bool host_only() const { return device < 0; }
 
@@ -247,32 +250,42 @@ struct svo_mm {
   //    every slot but the root's loses one visibility, visible treelets get
   //    MAX_VISIBILITY, then the new treelets are inserted in order
   //    (insertTreeletIntoIncoreBuffer, evicting through freeIncorePosition).
+  // The visible sets (performAnalysePass's host half): `old` gets every
+  // seen treelet and its ancestors with the largest error, `asked` is sorted
+  // and capped.  Returns the number of seen slots that hold a treelet.
+  uint32_t visible_sets(const std::vector<std::pair<uint32_t, float>>& seen,
+                        std::vector<std::pair<uint32_t, float>>* asked, std::map<uint32_t, float>* old) const {
+    uint32_t visible_slots = 0;
+    for (const auto& e : seen) {
+      if (e.first >= num_slots || slots[e.first].gid < 0) continue;
+      ++visible_slots;
+      uint32_t g = (uint32_t)slots[e.first].gid;
+      while (g != 0) {
+        auto it = old->find(g);
+        if (it == old->end()) (*old)[g] = e.second;
+        else if (e.second > it->second) it->second = e.second;
+        g = scene->treelets[g].h.parent_gid;
+      }
+      (*old)[0] = 99.0f;
+    }
+    std::stable_sort(asked->begin(), asked->end(), [](const std::pair<uint32_t, float>& a, const std::pair<uint32_t, float>& b) {
+      return a.first < b.first;
+    });
+    std::stable_sort(asked->begin(), asked->end(), [](const std::pair<uint32_t, float>& a, const std::pair<uint32_t, float>& b) {
+      return a.second > b.second;
+    });
+    const uint32_t cap = (uint32_t)((12.0 * 1048576.0f) / (double)scene->treelet_bytes);
+    if (asked->size() > cap) asked->resize(cap);
+    return visible_slots;
+  }
+
   void apply_records(const std::vector<std::pair<uint32_t, float>>& seen,
                      std::vector<std::pair<uint32_t, float>> asked, svo_feedback_stats* st) {
     svo_feedback_stats S{};
     std::map<uint32_t, float> old;
-    for (const auto& e : seen) {
-      if (e.first >= num_slots || slots[e.first].gid < 0) continue;
-      ++S.visible_slots;
-      uint32_t g = (uint32_t)slots[e.first].gid;
-      while (g != 0) {
-        auto it = old.find(g);
-        if (it == old.end()) old[g] = e.second;
-        else if (e.second > it->second) it->second = e.second;
-        g = scene->treelets[g].h.parent_gid;
-      }
-      old[0] = 99.0f;
-    }
-    S.visible_treelets = (uint32_t)old.size();
     S.requests = (uint32_t)asked.size();
-    std::stable_sort(asked.begin(), asked.end(), [](const std::pair<uint32_t, float>& a, const std::pair<uint32_t, float>& b) {
-      return a.first < b.first;
-    });
-    std::stable_sort(asked.begin(), asked.end(), [](const std::pair<uint32_t, float>& a, const std::pair<uint32_t, float>& b) {
-      return a.second > b.second;
-    });
-    const uint32_t cap = (uint32_t)((12.0 * 1048576.0f) / (double)scene->treelet_bytes);
-    if (asked.size() > cap) asked.resize(cap);
+    S.visible_slots = visible_sets(seen, &asked, &old);
+    S.visible_treelets = (uint32_t)old.size();
     S.requests_kept = (uint32_t)asked.size();
     for (uint32_t i = 1; i < num_slots; ++i) --slots[i].visibility;
     for (const auto& e : old) {
@@ -534,6 +547,7 @@ extern "C" void svo_mm_destroy(svo_mm* mm) { delete mm; }
 extern "C" svo_status svo_mm_reset(svo_mm* mm) {
   if (!mm) { set_error("svo_mm_reset: null handle"); return SVO_ERR_ARG; }
   mm->reset_host();
+  mm->fb_cached = false;
   for (uint32_t s = 0; s < mm->num_slots; ++s) mm->mark(s);
   return mm->insert(0, 999.0f);
 }
@@ -797,11 +811,12 @@ extern "C" svo_status svo_mm_apply_records(svo_mm* mm, const uint32_t* slots, co
   return SVO_OK;
 }
 
-extern "C" svo_status svo_mm_apply_feedback(svo_mm* mm, void* stream, svo_feedback_stats* stats) {
-  if (!mm) { set_error("svo_mm_apply_feedback: null handle"); return SVO_ERR_ARG; }
-  if (mm->host_only()) { set_error("svo_mm_apply_feedback: handle has no device pools"); return SVO_ERR_STATE; }
+namespace {
+
+// Reads this cycle's slot keys and asked gids back into mm->fb_seen /
+// fb_asked, then starts a new recording cycle.
+svo_status read_feedback_tables(svo_mm* mm, hipStream_t st) {
   HIP_TRY(hipSetDevice(mm->device));
-  hipStream_t st = (hipStream_t)stream;
   const uint32_t ng = (uint32_t)mm->scene->treelets.size();
   HIP_TRY(hipMemcpyAsync(mm->h_fb_slot, mm->d_fb_slot, (size_t)mm->num_slots * 8, hipMemcpyDeviceToHost, st));
   HIP_TRY(hipMemcpyAsync(mm->h_fb_count, mm->d_fb_count, 4, hipMemcpyDeviceToHost, st));
@@ -815,14 +830,56 @@ extern "C" svo_status svo_mm_apply_feedback(svo_mm* mm, void* stream, svo_feedba
     HIP_TRY(hipStreamSynchronize(st));
   }
   const uint32_t stamp = mm->fb_stamp;
-  std::vector<std::pair<uint32_t, float>> seen, asked;
+  mm->fb_seen.clear();
+  mm->fb_asked.clear();
   for (uint32_t s = 0; s < mm->num_slots; ++s) {
     const unsigned long long k = mm->h_fb_slot[s];
-    if ((uint32_t)(k >> 32) == stamp) seen.push_back({s, key_error(k)});
+    if ((uint32_t)(k >> 32) == stamp) mm->fb_seen.push_back({s, key_error(k)});
   }
-  for (uint32_t i = 0; i < n; ++i) asked.push_back({mm->h_fb_queue[i], key_error(mm->h_fb_out[i])});
+  for (uint32_t i = 0; i < n; ++i) mm->fb_asked.push_back({mm->h_fb_queue[i], key_error(mm->h_fb_out[i])});
   ++mm->fb_stamp;  // records of this cycle are now stale
-  mm->apply_records(seen, asked, stats);
+  return SVO_OK;
+}
+
+}  // namespace
+
+extern "C" svo_status svo_mm_read_feedback(svo_mm* mm, void* stream, uint32_t* old_gids, float* old_errors,
+                                           uint32_t old_cap, uint32_t* n_old, uint32_t* new_gids, float* new_errors,
+                                           uint32_t new_cap, uint32_t* n_new) {
+  if (!mm || !n_old || !n_new || (old_cap && (!old_gids || !old_errors)) || (new_cap && (!new_gids || !new_errors))) {
+    set_error("svo_mm_read_feedback: null argument");
+    return SVO_ERR_ARG;
+  }
+  if (mm->host_only()) { set_error("svo_mm_read_feedback: handle has no device pools"); return SVO_ERR_STATE; }
+  if (!mm->fb_cached) {
+    svo_status r = read_feedback_tables(mm, (hipStream_t)stream);
+    if (r != SVO_OK) return r;
+    mm->fb_cached = true;
+  }
+  std::vector<std::pair<uint32_t, float>> asked = mm->fb_asked;
+  std::map<uint32_t, float> old;
+  mm->visible_sets(mm->fb_seen, &asked, &old);
+  uint32_t i = 0;
+  for (const auto& e : old) {
+    if (i < old_cap) { old_gids[i] = e.first; old_errors[i] = e.second; }
+    ++i;
+  }
+  *n_old = i;
+  for (i = 0; i < asked.size(); ++i)
+    if (i < new_cap) { new_gids[i] = asked[i].first; new_errors[i] = asked[i].second; }
+  *n_new = (uint32_t)asked.size();
+  return SVO_OK;
+}
+
+extern "C" svo_status svo_mm_apply_feedback(svo_mm* mm, void* stream, svo_feedback_stats* stats) {
+  if (!mm) { set_error("svo_mm_apply_feedback: null handle"); return SVO_ERR_ARG; }
+  if (mm->host_only()) { set_error("svo_mm_apply_feedback: handle has no device pools"); return SVO_ERR_STATE; }
+  if (!mm->fb_cached) {
+    svo_status r = read_feedback_tables(mm, (hipStream_t)stream);
+    if (r != SVO_OK) return r;
+  }
+  mm->fb_cached = false;
+  mm->apply_records(mm->fb_seen, mm->fb_asked, stats);
   return SVO_OK;
 }
 
