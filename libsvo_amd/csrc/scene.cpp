@@ -209,7 +209,7 @@ svo_status build_scene(uint32_t depth, std::vector<uint64_t>& codes,
 
   uint32_t tb = treelet_bytes;
   if (tb == 0) {
-    uint64_t need = (tree_nodes + 9) * 8;
+    uint64_t need = std::max<uint64_t>((tree_nodes + 9) * 8, 128);  // the .svo reader's minimum
     if (need > 0xFFFFFFF8ull) { set_error("tree of %llu nodes does not fit one treelet", (unsigned long long)tree_nodes); return SVO_ERR_ARG; }
     tb = (uint32_t)need;
   }
@@ -228,7 +228,8 @@ svo_status build_scene(uint32_t depth, std::vector<uint64_t>& codes,
   // at an odd level cuts among its grandchildren instead.
   std::vector<std::vector<uint8_t>> cut(depth + 1);
   if (!bfs) {
-    if (N < 80) { set_error("dense packing needs treelets of at least 80 nodes (got %u)", N); return SVO_ERR_ARG; }
+    // (a tree that fits one treelet needs no cut, whatever N)
+    if (N < 80 && tree_nodes > N) { set_error("dense packing needs treelets of at least 80 nodes (got %u)", N); return SVO_ERR_ARG; }
     std::vector<std::vector<uint32_t>> w(depth + 1);
     w[depth].assign(lv[depth].attr.size() / 2, 1u);
     cut[depth].assign(w[depth].size(), 0);

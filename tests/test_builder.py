@@ -199,3 +199,16 @@ def test_spheres_match_restatement():
         r = 0.05 + (0.2 - 0.05) * u(s, 3)
         filled |= (X - cx) ** 2 + (Y - cy) ** 2 + (Z - cz) ** 2 < r * r
     assert sorted(got) == leaves_from_mask(shell_of(filled), depth)
+
+
+def test_tiny_tree_in_one_treelet(tmp_path):
+    # a tree that fits one treelet needs no cut, however small the treelet
+    code = H.morton_np(np.array([5]), np.array([2]), np.array([6]), 3)
+    sc = svo.Scene.from_leaves(3, code, np.array([[0x80FF8000, 0x11223344]], np.uint32), 0, "dense")
+    assert sc.info["num_treelets"] == 1 and sc.info["tree_nodes"] == 4 and sc.info["tree_leaves"] == 1
+    assert sc.info["treelet_bytes"] == 128  # the smallest treelet a .svo file holds
+    check_invariants(sc, "dense")
+    assert H.walk_leaves(H.treelets(sc), 3) == {int(code[0]): 0x11223344}
+    sc.write(str(tmp_path / "tiny"))
+    back = svo.Scene.open(str(tmp_path / "tiny"))
+    assert back.info == sc.info
