@@ -474,8 +474,10 @@ svo_status svo_mm_apply_feedback(svo_mm* mm, void* stream,
  *   new: the treelets asked for, by descending error (ties: ascending gid),
  *        capped at 12 MiB of treelets.
  * Reads the tables back on `stream` and ends the recording cycle (later
- * recordings start a new one); the records read are kept, and the next
- * svo_mm_apply_feedback applies them instead of reading the tables.  Up to
+ * recordings start a new one).  The sets are computed against the slot
+ * table of this moment and kept; the next svo_mm_apply_feedback applies
+ * them instead of reading the tables (as performAnalysePass computes them
+ * before updateClientSideIncoreBuffer applies them).  Up to
  * old_cap / new_cap entries are written; *n_old / *n_new receive the full
  * counts (arrays may be null when their cap is 0). */
 svo_status svo_mm_read_feedback(svo_mm* mm, void* stream,

@@ -126,7 +126,8 @@ struct svo_mm {
   UploadSpan* d_spans = nullptr;
   uint32_t span_cap = 0;
   int32_t* h_slot_tab = nullptr;
-  // records read by svo_mm_read_feedback, applied by the next apply
+  // the records read back by the last table read, and whether the sets made
+  // from them (fb_sets) wait for the next apply
   bool fb_cached = false;
   std::vector<std::pair<uint32_t, float>> fb_seen, fb_asked;
 
@@ -279,22 +280,39 @@ struct svo_mm {
     return visible_slots;
   }
 
-  void apply_records(const std::vector<std::pair<uint32_t, float>>& seen,
-                     std::vector<std::pair<uint32_t, float>> asked, svo_feedback_stats* st) {
-    svo_feedback_stats S{};
+  // The sets one analysis cycle yields, computed against the slot table of
+  // the moment (as performAnalysePass does), applied later.
+  struct FeedbackSets {
     std::map<uint32_t, float> old;
-    S.requests = (uint32_t)asked.size();
-    S.visible_slots = visible_sets(seen, &asked, &old);
-    S.visible_treelets = (uint32_t)old.size();
-    S.requests_kept = (uint32_t)asked.size();
+    std::vector<std::pair<uint32_t, float>> asked;  // sorted, capped
+    uint32_t visible_slots = 0;
+    uint32_t requests = 0;                          // before the cap
+  };
+  FeedbackSets fb_sets;  // computed by svo_mm_read_feedback (valid while fb_cached)
+
+  FeedbackSets make_sets(const std::vector<std::pair<uint32_t, float>>& seen,
+                         std::vector<std::pair<uint32_t, float>> asked) const {
+    FeedbackSets F;
+    F.requests = (uint32_t)asked.size();
+    F.visible_slots = visible_sets(seen, &asked, &F.old);
+    F.asked = std::move(asked);
+    return F;
+  }
+
+  void apply_sets(const FeedbackSets& F, svo_feedback_stats* st) {
+    svo_feedback_stats S{};
+    S.requests = F.requests;
+    S.visible_slots = F.visible_slots;
+    S.visible_treelets = (uint32_t)F.old.size();
+    S.requests_kept = (uint32_t)F.asked.size();
     for (uint32_t i = 1; i < num_slots; ++i) --slots[i].visibility;
-    for (const auto& e : old) {
+    for (const auto& e : F.old) {
       const int32_t sl = treelet_slot[e.first];
       if (sl >= 0) slots[sl].visibility = kMaxVisibility;
     }
     const uint64_t ev0 = evictions;
     deferring = true;
-    for (const auto& e : asked) {
+    for (const auto& e : F.asked) {
       if (insert(e.first, e.second) == SVO_OK) ++S.inserted;
       else ++S.insert_failed;
     }
@@ -303,6 +321,11 @@ struct svo_mm {
     S.evicted = (uint32_t)(evictions - ev0);
     S.resident_treelets = resident;
     if (st) *st = S;
+  }
+
+  void apply_records(const std::vector<std::pair<uint32_t, float>>& seen,
+                     std::vector<std::pair<uint32_t, float>> asked, svo_feedback_stats* st) {
+    apply_sets(make_sets(seen, std::move(asked)), st);
   }
 
   // insertTreeletIntoIncoreBuffer (src/TreeletMemoryManager.cpp:455-548).
@@ -854,20 +877,19 @@ extern "C" svo_status svo_mm_read_feedback(svo_mm* mm, void* stream, uint32_t* o
   if (!mm->fb_cached) {
     svo_status r = read_feedback_tables(mm, (hipStream_t)stream);
     if (r != SVO_OK) return r;
+    mm->fb_sets = mm->make_sets(mm->fb_seen, mm->fb_asked);
     mm->fb_cached = true;
   }
-  std::vector<std::pair<uint32_t, float>> asked = mm->fb_asked;
-  std::map<uint32_t, float> old;
-  mm->visible_sets(mm->fb_seen, &asked, &old);
+  const svo_mm::FeedbackSets& F = mm->fb_sets;
   uint32_t i = 0;
-  for (const auto& e : old) {
+  for (const auto& e : F.old) {
     if (i < old_cap) { old_gids[i] = e.first; old_errors[i] = e.second; }
     ++i;
   }
   *n_old = i;
-  for (i = 0; i < asked.size(); ++i)
-    if (i < new_cap) { new_gids[i] = asked[i].first; new_errors[i] = asked[i].second; }
-  *n_new = (uint32_t)asked.size();
+  for (i = 0; i < F.asked.size(); ++i)
+    if (i < new_cap) { new_gids[i] = F.asked[i].first; new_errors[i] = F.asked[i].second; }
+  *n_new = (uint32_t)F.asked.size();
   return SVO_OK;
 }
 
@@ -877,9 +899,10 @@ extern "C" svo_status svo_mm_apply_feedback(svo_mm* mm, void* stream, svo_feedba
   if (!mm->fb_cached) {
     svo_status r = read_feedback_tables(mm, (hipStream_t)stream);
     if (r != SVO_OK) return r;
+    mm->fb_sets = mm->make_sets(mm->fb_seen, mm->fb_asked);
   }
   mm->fb_cached = false;
-  mm->apply_records(mm->fb_seen, mm->fb_asked, stats);
+  mm->apply_sets(mm->fb_sets, stats);
   return SVO_OK;
 }
 
