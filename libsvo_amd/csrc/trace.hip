@@ -743,7 +743,18 @@ __global__ __launch_bounds__(64 * WAVES) void trace_kernel(const TraceFrame* __r
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = tid & 63u;
   const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const uint32_t tile = blockIdx.x * WAVES + wave;
+  // Workgroups are dispatched round-robin over the 8 XCDs (b -> XCD b % 8),
+  // each with its own L2.  Renumber them so that every XCD gets runs of
+  // kXcdRun neighbouring tiles, whose rays share the upper levels of the
+  // tree, instead of every 8th tile (a bijection on each full group of
+  // 8 * kXcdRun workgroups; the last partial group keeps its order).
+  constexpr uint32_t kXcdRun = 32u;
+  uint32_t wg = blockIdx.x;
+  if (wg < gridDim.x / (8u * kXcdRun) * (8u * kXcdRun)) {
+    const uint32_t g = wg / (8u * kXcdRun), r = wg % (8u * kXcdRun);
+    wg = g * (8u * kXcdRun) + (r % 8u) * kXcdRun + r / 8u;
+  }
+  const uint32_t tile = wg * WAVES + wave;
   if (tile >= num_tiles) return;
 
   uint32_t lo = 0, hi = num_frames;

@@ -80,6 +80,11 @@ def test_analyse_apply_render_matches_reference_bookkeeping(terrain8, policy):
         assert st["inserted"] == inserted and st["evicted"] == model.evictions - ev0, f
         assert mm.getSlots() == model.slot_gid, f"slot tables differ after frame {f}"
         mm.updateDeviceMemory(s)
+        if f % 4 == 3:  # the upload kernel left HBM equal to the host incore buffer
+            n = mm.info["pool_nodes"]
+            hn, ha = mm.getIncoreBuffer(0, n)
+            dn, da = mm.getDeviceIncoreBuffer(0, n)
+            assert np.array_equal(hn, dn) and np.array_equal(ha, da), f"device incore differs after frame {f}"
         gpu = svo.render(mm, cam, policy)
         _same(gpu, _oracle(mm, cam, policy), f"frame {f}")
         pages += inserted
