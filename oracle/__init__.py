@@ -1,9 +1,9 @@
 """CPU oracle of the serialized-SVO ray caster (test infrastructure).
 
 Loads ``oracle/libsvo_oracle.so`` (built from ``oracle/svo_oracle.cpp`` by
-``make -C oracle``).  Only the tests, ``__graft_entry__.smoke()`` (as the
-checker) and ``bench.py``'s CPU-baseline leg use this package; the product
-library never does.
+``make -C oracle``).  Only the tests, ``__graft_entry__.smoke()`` (to check
+the GPU result) and ``bench.py``'s CPU-baseline leg use this package; the
+product library never does.
 """
 from __future__ import annotations
 

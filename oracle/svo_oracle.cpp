@@ -2,8 +2,8 @@
 //
 // TEST INFRASTRUCTURE ONLY.  This file restates the reference's algorithms on
 // the CPU so that the HIP path can be checked against them; it is loaded by
-// the tests, by __graft_entry__.smoke() as the checker and by bench.py's
-// CPU-baseline leg, and never by the product library.
+// the tests, by __graft_entry__.smoke() to check the GPU result and by
+// bench.py's CPU-baseline leg, and never by the product library.
 //
 // It is written independently of libsvo_amd/csrc: its own .svo/.ia reader,
 // its own incore-buffer stitching and a traversal that keeps the reference's
