@@ -10,7 +10,9 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libsvo_hip.so")
+# SVO_HIP_LIBRARY may name another build of the same library (the
+# sanitizer build, libsvo_amd/_san/libsvo_hip.so)
+LIB_PATH = os.environ.get("SVO_HIP_LIBRARY") or os.path.join(_HERE, "libsvo_hip.so")
 
 u32, u64, i32, f32, f64 = ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int32, ctypes.c_float, ctypes.c_double
 vp = ctypes.c_void_p

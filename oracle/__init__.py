@@ -14,7 +14,7 @@ from typing import Dict, Optional
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libsvo_oracle.so")
+LIB_PATH = os.environ.get("SVO_ORACLE_LIBRARY") or os.path.join(_HERE, "libsvo_oracle.so")
 
 if not os.path.exists(LIB_PATH):
     raise ImportError(f"{LIB_PATH} is not built; run make -C oracle")
