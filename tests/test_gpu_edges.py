@@ -143,3 +143,30 @@ def test_single_leaf_scene(tmp_path):
                 assert np.all(gpu["rgba"][hits] == 0x112233FF)
             else:
                 assert np.all(gpu["rgba"] == svo.BACKGROUND_RGBA)
+
+
+@pytest.mark.parametrize("policy", ["treelet", "cpu3"])
+def test_far_camera_maxloops_rays(tmp_path, policy):
+    """A camera 2000 units away drives rays into maxLoops (the epsilon
+    vanishes against t): pixels, push counts and loop counts (the loops view
+    mode) match the oracle, including the rays whose cycles the kernel skips
+    by whole periods."""
+    import oracle
+    from libsvo_amd import workloads
+    sc = workloads.menger_scene(depth=6, level=3, treelet_bytes=4096)
+    base = str(tmp_path / "menger6")
+    sc.write(base)
+    ib = workloads.incore_bytes_for(sc)
+    mm = svo.TreeletMemoryManagerHip(sc, ib, 0, resident_all=True)
+    ora = oracle.OracleScene(base, ib)
+    eye = np.array([0.7, 0.45, 0.55])
+    eye = eye / np.linalg.norm(eye) * 2000.0
+    cam = svo.with_view_mode(svo.look_at(tuple(eye), (0.01, 0.02, 0.03), (0.0, 1.0, 0.0), 0.09, 64, 64,
+                                         far=4000.0), "loops")
+    gpu = svo.render(mm, cam, policy, shade=True)
+    ref = ora.render(cam, policy, shade=True)
+    _same(gpu, ref, f"far camera {policy}")
+    assert np.array_equal(gpu["pushes"], ref["pushes"])
+    assert np.array_equal(gpu["shade"].view(np.uint32), ref["shade"].view(np.uint32))
+    max_loops = 14 * 14 * 5 if policy == "treelet" else 7 * 7 * 5
+    assert int((ref["loops"] == max_loops).sum()) > 40
