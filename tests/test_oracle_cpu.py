@@ -9,8 +9,11 @@ What each oracle function is checked against:
                              C except on flagged boundary rays; hit point on
                              the hit voxel's box
   pixel_ray                  finite, unit length, through the far-plane pixel
-  maxLoops exits             not reachable by a constructed scene here:
-                             parity unpinned for that branch
+  maxLoops exits             a pure-Python restatement of both traversals on
+                             far-camera rays that cycle to maxLoops
+                             (test_maxloops.py)
+  shade (view modes)         numpy formulas per mode (test_view_modes.py)
+  sample_fill (policy B)     brute-force first hit (test_single_svo.py)
 """
 import math
 
