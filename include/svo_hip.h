@@ -517,6 +517,11 @@ svo_status svo_mm_apply_records(svo_mm* mm, const uint32_t* slots,
  *    views and max_batch_rays rays (at least one request).
  *    max_inflight = max_batch_views = 1 is the plain one-frame-per-launch
  *    loop.
+ *  - Overload: with max_waiting > 0 at most that many requests wait for a
+ *    launch; svo_server_submit refuses more with SVO_ERR_CAPACITY (replay
+ *    completes such a request as SVO_REQ_REJECTED).  A request with
+ *    timeout_us > 0 that is still waiting that long after its arrival is
+ *    dropped unrendered and completes as SVO_REQ_EXPIRED.
  * A server is driven from one host thread. */
 typedef struct svo_server svo_server;
 
@@ -527,12 +532,12 @@ typedef struct svo_server_config {
   uint32_t max_batch_views;    /* 1..65536 */
   uint32_t max_inflight;       /* 1..16 */
   int32_t policy;              /* SVO_POLICY_* */
-  uint32_t reserved;
+  uint32_t max_waiting;        /* requests waiting for a launch; 0 = no limit */
 } svo_server_config;
 
 typedef struct svo_request {
   uint32_t scene;              /* id from svo_server_add_scene */
-  uint32_t reserved;
+  uint32_t timeout_us;         /* longest wait for a launch; 0 = none */
   double arrival_s;            /* svo_server_replay: offset from the start */
   svo_camera camera;
   svo_frame_out out;           /* all null: outputs from the frame pool */
@@ -540,11 +545,15 @@ typedef struct svo_request {
 
 /* Times in seconds since the server was created (online use) or since the
  * start of the replay. */
+enum { SVO_REQ_DONE = 0, SVO_REQ_EXPIRED = 1, SVO_REQ_REJECTED = 2 };
+
 typedef struct svo_completion {
   uint64_t id;                 /* svo_server_submit id / replay index */
-  double arrival_s, launch_s, done_s;
+  double arrival_s, launch_s, done_s;   /* not rendered: launch_s = done_s */
   uint32_t batch;              /* launch sequence number */
-  uint32_t batch_views;        /* views in that launch */
+  uint32_t batch_views;        /* views in that launch (0: not rendered) */
+  uint32_t status;             /* SVO_REQ_* */
+  uint32_t reserved;
 } svo_completion;
 
 typedef struct svo_server_stats {
@@ -553,8 +562,9 @@ typedef struct svo_server_stats {
   uint32_t max_batch_views_seen, max_inflight_seen;
   double wall_s;               /* first arrival to last completion */
   double frames_per_s, mrays_per_s;
-  double p50_ms, p99_ms, mean_ms, max_ms;   /* done - arrival */
+  double p50_ms, p99_ms, mean_ms, max_ms;   /* done - arrival, rendered */
   double mean_batch_views;
+  uint64_t expired, rejected;  /* not rendered (frames_per_s counts the rest) */
 } svo_server_stats;
 
 svo_status svo_server_create(svo_renderer* r, const svo_server_config* cfg,

@@ -562,12 +562,12 @@ class Server:
     (results stay in HBM); requests with one render into it."""
 
     def __init__(self, renderer: Renderer, hbm_budget: int = 0, frame_pool: int = 1 << 30, max_batch_rays: int = 0,
-                 max_batch_views: int = 64, max_inflight: int = 4, policy="treelet"):
+                 max_batch_views: int = 64, max_inflight: int = 4, policy="treelet", max_waiting: int = 0):
         self._renderer = renderer
         self._scenes: List[TreeletMemoryManagerHip] = []
         self._frames = {}
         cfg = _capi.svo_server_config(int(hbm_budget), int(frame_pool), int(max_batch_rays), int(max_batch_views),
-                                      int(max_inflight), _policy(policy), 0)
+                                      int(max_inflight), _policy(policy), int(max_waiting))
         self._h = ctypes.c_void_p()
         check(lib.svo_server_create(renderer.handle, ctypes.byref(cfg), ctypes.byref(self._h)), "svo_server_create")
 
@@ -583,13 +583,16 @@ class Server:
         return sid.value
 
     @staticmethod
-    def _request(scene: int, cam: svo_camera, arrival_s: float = 0.0, frame: Optional[Frame] = None):
+    def _request(scene: int, cam: svo_camera, arrival_s: float = 0.0, frame: Optional[Frame] = None,
+                 timeout_us: int = 0):
         out = frame.out() if frame is not None else svo_frame_out()
-        return _capi.svo_request(scene, 0, float(arrival_s), cam, out)
+        return _capi.svo_request(scene, int(timeout_us), float(arrival_s), cam, out)
 
-    def submit(self, scene: int, cam: svo_camera, frame: Optional[Frame] = None) -> int:
+    def submit(self, scene: int, cam: svo_camera, frame: Optional[Frame] = None, timeout_us: int = 0) -> int:
+        """Queues a request (SvoError with status SVO_ERR_CAPACITY when
+        max_waiting requests already wait)."""
         rid = ctypes.c_uint64()
-        req = self._request(scene, cam, 0.0, frame)
+        req = self._request(scene, cam, 0.0, frame, timeout_us)
         check(lib.svo_server_submit(self._h, ctypes.byref(req), ctypes.byref(rid)), "svo_server_submit")
         if frame is not None:
             self._frames[rid.value] = frame  # kept alive until completion
@@ -612,9 +615,10 @@ class Server:
         return n.value
 
     def replay(self, requests: Sequence[tuple]):
-        """Serves (scene, camera, arrival_s[, frame]) requests in real time.
-        Returns (stats dict, completions array with fields arrival_s,
-        launch_s, done_s, batch, batch_views, in request order)."""
+        """Serves (scene, camera, arrival_s[, frame[, timeout_us]]) requests
+        in real time.  Returns (stats dict, completions array with fields
+        arrival_s, launch_s, done_s, batch, batch_views, status, in request
+        order)."""
         n = len(requests)
         reqs = (_capi.svo_request * n)(*[self._request(*r) for r in requests])
         done = (_capi.svo_completion * n)()

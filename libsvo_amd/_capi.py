@@ -22,6 +22,7 @@ SVO_OK = 0
 SVO_ERR_ARG, SVO_ERR_IO, SVO_ERR_FORMAT, SVO_ERR_HIP, SVO_ERR_NOMEM, SVO_ERR_STATE, SVO_ERR_CAPACITY = range(1, 8)
 SVO_POLICY_TREELET = 0
 SVO_POLICY_SINGLE = 1
+SVO_REQ_DONE, SVO_REQ_EXPIRED, SVO_REQ_REJECTED = 0, 1, 2
 SVO_POLICY_CPU3 = 2
 SVO_BUILD_PACK_BFS = 1
 SVO_SINGLE_SVOLOADER, SVO_SINGLE_V1 = 1, 2
@@ -77,24 +78,25 @@ class svo_feedback_stats(ctypes.Structure):
 
 class svo_server_config(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", u64), ("frame_pool_bytes", u64), ("max_batch_rays", u64),
-                ("max_batch_views", u32), ("max_inflight", u32), ("policy", i32), ("reserved", u32)]
+                ("max_batch_views", u32), ("max_inflight", u32), ("policy", i32), ("max_waiting", u32)]
 
 
 class svo_request(ctypes.Structure):
-    _fields_ = [("scene", u32), ("reserved", u32), ("arrival_s", f64), ("camera", svo_camera),
+    _fields_ = [("scene", u32), ("timeout_us", u32), ("arrival_s", f64), ("camera", svo_camera),
                 ("out", svo_frame_out)]
 
 
 class svo_completion(ctypes.Structure):
     _fields_ = [("id", u64), ("arrival_s", f64), ("launch_s", f64), ("done_s", f64), ("batch", u32),
-                ("batch_views", u32)]
+                ("batch_views", u32), ("status", u32), ("reserved", u32)]
 
 
 class svo_server_stats(ctypes.Structure):
     _fields_ = [("requests", u64), ("rays", u64), ("batches", u64), ("peak_pool_bytes", u64),
                 ("max_batch_views_seen", u32), ("max_inflight_seen", u32), ("wall_s", f64),
                 ("frames_per_s", f64), ("mrays_per_s", f64), ("p50_ms", f64), ("p99_ms", f64),
-                ("mean_ms", f64), ("max_ms", f64), ("mean_batch_views", f64)]
+                ("mean_ms", f64), ("max_ms", f64), ("mean_batch_views", f64), ("expired", u64),
+                ("rejected", u64)]
 
 
 class svo_launch_config(ctypes.Structure):

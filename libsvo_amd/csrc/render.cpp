@@ -254,7 +254,7 @@ extern "C" svo_status svo_autotune(svo_renderer* r, const svo_view* views, uint3
 // ------------------------------------------------------------ batches
 
 struct svo_batch {
-  svo_renderer* r = nullptr;
+  int device = 0;  // the renderer's at creation (the batch does not keep the renderer)
   int policy = 0;
   LaunchShape shape{};
   StagedLaunch sg;
@@ -268,7 +268,7 @@ struct svo_batch {
   ~svo_batch() {
     int cur = 0;
     hipGetDevice(&cur);
-    hipSetDevice(r->device);
+    hipSetDevice(device);
     if (pending) hipEventSynchronize(copied);
     hipDeviceSynchronize();
     if (exec) hipGraphExecDestroy(exec);
@@ -286,7 +286,7 @@ extern "C" svo_status svo_batch_create(svo_renderer* r, const svo_view* views, u
   *out = nullptr;
   HIP_TRY(hipSetDevice(r->device));
   std::unique_ptr<svo_batch> b(new svo_batch());
-  b->r = r;
+  b->device = r->device;
   b->policy = policy;
   b->shape = r->shape;
   svo_status st = stage_views(views, n, policy, b->shape, r->device, &b->sg);
@@ -322,7 +322,7 @@ extern "C" svo_status svo_batch_set_cameras(svo_batch* b, const svo_camera* cams
       set_error("camera %u changes the frame size of the batch", i);
       return SVO_ERR_ARG;
     }
-  HIP_TRY(hipSetDevice(b->r->device));
+  HIP_TRY(hipSetDevice(b->device));
   if (b->pending) {
     HIP_TRY(hipEventSynchronize(b->copied));
     b->pending = false;
@@ -336,7 +336,7 @@ extern "C" svo_status svo_batch_set_cameras(svo_batch* b, const svo_camera* cams
 
 extern "C" svo_status svo_batch_launch(svo_batch* b, void* stream) {
   if (!b) { set_error("svo_batch_launch: null batch"); return SVO_ERR_ARG; }
-  HIP_TRY(hipSetDevice(b->r->device));
+  HIP_TRY(hipSetDevice(b->device));
   HIP_TRY(hipGraphLaunch(b->exec, (hipStream_t)stream));
   return SVO_OK;
 }

@@ -113,7 +113,8 @@ uint64_t frame_bytes(const svo_camera& c) { return (uint64_t)c.width * c.height 
 }  // namespace
 
 struct svo_server {
-  svo_renderer* r = nullptr;
+  svo_renderer* r = nullptr;  // launches take its current launch shape
+  int device = 0;             // the renderer's, kept so that teardown never reads the renderer
   svo_server_config cfg{};
   std::vector<svo_mm*> scenes;
   uint64_t scene_bytes = 0;
@@ -125,13 +126,14 @@ struct svo_server {
   uint32_t batches = 0;
   uint32_t max_views_seen = 0, max_inflight_seen = 0;
   uint64_t views_launched = 0;
+  uint64_t expired = 0;
   Clock::time_point t0 = Clock::now();
   std::vector<svo_view> views;   // scratch
   StagedLaunch staged;           // scratch
   std::vector<svo_completion> held;  // completions not yet returned by poll
 
   ~svo_server() {
-    hipSetDevice(renderer_device(r));
+    hipSetDevice(device);
     for (Flight& f : flights) {
       if (f.stream) hipStreamSynchronize(f.stream);
       if (f.done) hipEventDestroy(f.done);
@@ -156,6 +158,10 @@ struct svo_server {
     if (c.width == 0 || c.height == 0 || c.width > 16384 || c.height > 16384) {
       set_error("request frame %ux%u out of range", c.width, c.height);
       return SVO_ERR_ARG;
+    }
+    if (cfg.max_waiting && waiting.size() >= cfg.max_waiting) {
+      set_error("server queue full (%u requests waiting)", cfg.max_waiting);
+      return SVO_ERR_CAPACITY;
     }
     Request r;
     r.id = next_id++;
@@ -186,7 +192,8 @@ struct svo_server {
       const double t = now();
       for (const Request& q : f.reqs) {
         if (q.pooled) arena.release(q.pool_off, q.pool_bytes);
-        out->push_back(svo_completion{q.id, q.arrival_s, q.launch_s, t, f.batch, (uint32_t)f.reqs.size()});
+        out->push_back(svo_completion{q.id, q.arrival_s, q.launch_s, t, f.batch, (uint32_t)f.reqs.size(),
+                                      SVO_REQ_DONE, 0});
       }
       f.reqs.clear();
       f.busy = false;
@@ -226,7 +233,7 @@ struct svo_server {
         }
         views[k] = svo_view{scenes[q.req.scene], q.req.camera, o};
       }
-      svo_status st = stage_views(views.data(), (uint32_t)views.size(), cfg.policy, shape, renderer_device(r), &staged);
+      svo_status st = stage_views(views.data(), (uint32_t)views.size(), cfg.policy, shape, device, &staged);
       if (st != SVO_OK) {  // e.g. a scene with slots not uploaded: the requests wait
         for (auto it = f.reqs.rbegin(); it != f.reqs.rend(); ++it) {
           if (it->pooled) arena.release(it->pool_off, it->pool_bytes);
@@ -262,9 +269,24 @@ struct svo_server {
     return SVO_OK;
   }
 
+  // Waiting requests past their timeout complete unrendered.
+  void expire(std::vector<svo_completion>* out) {
+    const double t = now();
+    for (auto it = waiting.begin(); it != waiting.end();) {
+      if (it->req.timeout_us && (t - it->arrival_s) * 1e6 > (double)it->req.timeout_us) {
+        out->push_back(svo_completion{it->id, it->arrival_s, t, t, 0, 0, SVO_REQ_EXPIRED, 0});
+        ++expired;
+        it = waiting.erase(it);
+      } else {
+        ++it;
+      }
+    }
+  }
+
   svo_status progress(std::vector<svo_completion>* out) {
     svo_status st = retire(out);
     if (st != SVO_OK) return st;
+    expire(out);
     return launch_ready();
   }
 };
@@ -292,6 +314,7 @@ extern "C" svo_status svo_server_create(svo_renderer* r, const svo_server_config
   HIP_TRY(hipSetDevice(renderer_device(r)));
   std::unique_ptr<svo_server> s(new svo_server());
   s->r = r;
+  s->device = renderer_device(r);
   s->cfg = *cfg;
   if (cfg->frame_pool_bytes) HIP_TRY(hipMalloc(&s->pool, cfg->frame_pool_bytes));
   s->arena.init(cfg->frame_pool_bytes);
@@ -316,8 +339,8 @@ extern "C" svo_status svo_server_add_scene(svo_server* s, svo_mm* mm, uint32_t* 
   svo_mm_info info;
   svo_status st = svo_mm_get_info(mm, &info);
   if (st != SVO_OK) return st;
-  if ((int)info.device != renderer_device(s->r) || info.device_bytes == 0) {
-    set_error("scene is not resident on the server's device %d", renderer_device(s->r));
+  if ((int)info.device != s->device || info.device_bytes == 0) {
+    set_error("scene is not resident on the server's device %d", s->device);
     return SVO_ERR_ARG;
   }
   const uint64_t need = s->scene_bytes + info.device_bytes + s->cfg.frame_pool_bytes;
@@ -340,7 +363,7 @@ extern "C" svo_status svo_server_submit(svo_server* s, const svo_request* req, u
 
 extern "C" svo_status svo_server_poll(svo_server* s, svo_completion* done, uint32_t cap, uint32_t* n_done) {
   if (!s || !n_done || (cap && !done)) { set_error("svo_server_poll: bad arguments"); return SVO_ERR_ARG; }
-  HIP_TRY(hipSetDevice(renderer_device(s->r)));
+  HIP_TRY(hipSetDevice(s->device));
   *n_done = 0;
   // completions beyond `cap` are held for the next poll
   std::vector<svo_completion>& held = s->held;
@@ -369,11 +392,13 @@ extern "C" svo_status svo_server_replay(svo_server* s, const svo_request* reqs, 
   if (pend) { set_error("svo_server_replay needs an idle server (%u requests pending)", pend); return SVO_ERR_STATE; }
   for (uint32_t i = 1; i < n; ++i)
     if (!(reqs[i].arrival_s >= reqs[i - 1].arrival_s)) { set_error("arrival_s must be ascending"); return SVO_ERR_ARG; }
-  HIP_TRY(hipSetDevice(renderer_device(s->r)));
+  HIP_TRY(hipSetDevice(s->device));
   for (Flight& f : s->flights)
     if (f.stream) HIP_TRY(hipStreamSynchronize(f.stream));  // outstanding caller work is not timed
 
   const uint64_t id0 = s->next_id;
+  const uint64_t e0 = s->expired;
+  uint64_t rejected = 0;
   const uint32_t b0 = s->batches;
   const uint64_t v0 = s->views_launched;
   s->max_views_seen = s->max_inflight_seen = 0;
@@ -387,7 +412,14 @@ extern "C" svo_status svo_server_replay(svo_server* s, const svo_request* reqs, 
     const double t = s->now();
     while (next < n && reqs[next].arrival_s <= t) {
       svo_status st = s->enqueue(reqs[next], reqs[next].arrival_s, nullptr);
-      if (st != SVO_OK) return st;
+      if (st == SVO_ERR_CAPACITY && s->cfg.max_waiting && s->waiting.size() >= s->cfg.max_waiting) {
+        // queue full: the request completes as rejected (its id is the one it would have had)
+        clear_error();
+        comp.push_back(svo_completion{s->next_id++, reqs[next].arrival_s, t, t, 0, 0, SVO_REQ_REJECTED, 0});
+        ++rejected;
+      } else if (st != SVO_OK) {
+        return st;
+      }
       ++next;
     }
     fresh.clear();
@@ -395,20 +427,24 @@ extern "C" svo_status svo_server_replay(svo_server* s, const svo_request* reqs, 
     if (st != SVO_OK) return st;
     comp.insert(comp.end(), fresh.begin(), fresh.end());
   }
-  std::vector<double> lat(n);
+  std::vector<double> lat;
+  lat.reserve(n);
   double first = n ? reqs[0].arrival_s : 0, last = 0, sum = 0;
   uint64_t rays = 0;
   for (const svo_completion& c : comp) {
     const uint64_t i = c.id - id0;
-    lat[i] = (c.done_s - c.arrival_s) * 1e3;
-    sum += lat[i];
+    if (c.status == SVO_REQ_DONE) {
+      lat.push_back((c.done_s - c.arrival_s) * 1e3);
+      sum += lat.back();
+      rays += (uint64_t)reqs[i].camera.width * reqs[i].camera.height;
+    }
     last = std::max(last, c.done_s);
-    rays += (uint64_t)reqs[i].camera.width * reqs[i].camera.height;
     if (done) {
       done[i] = c;
       done[i].id = i;
     }
   }
+  const uint64_t rendered = lat.size();
   if (stats) {
     svo_server_stats& S = *stats;
     S = svo_server_stats{};
@@ -419,10 +455,12 @@ extern "C" svo_status svo_server_replay(svo_server* s, const svo_request* reqs, 
     S.max_batch_views_seen = s->max_views_seen;
     S.max_inflight_seen = s->max_inflight_seen;
     S.wall_s = last - first;
-    if (n && S.wall_s > 0) {
-      S.frames_per_s = n / S.wall_s;
+    if (rendered && S.wall_s > 0) {
+      S.frames_per_s = rendered / S.wall_s;
       S.mrays_per_s = rays / S.wall_s / 1e6;
     }
+    S.expired = s->expired - e0;
+    S.rejected = rejected;
     std::vector<double> sorted = lat;
     std::sort(sorted.begin(), sorted.end());
     auto pct = [&](double p) {
@@ -433,7 +471,7 @@ extern "C" svo_status svo_server_replay(svo_server* s, const svo_request* reqs, 
     };
     S.p50_ms = pct(50);
     S.p99_ms = pct(99);
-    S.mean_ms = n ? sum / n : 0;
+    S.mean_ms = rendered ? sum / rendered : 0;
     S.max_ms = sorted.empty() ? 0 : sorted.back();
     S.mean_batch_views = S.batches ? (double)(s->views_launched - v0) / S.batches : 0;
   }

@@ -947,6 +947,7 @@ int launch_copy(void* dst, const void* src, size_t bytes, void* stream) {
   const size_t n = bytes / 16;
   if (n == 0) return 0;
   const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 256 * 64);
+  (void)hipGetLastError();  // see launch_trace
   hipLaunchKernelGGL(copy_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream, (uint4*)dst, (const uint4*)src, n);
   return (int)hipGetLastError();
 }
@@ -954,6 +955,7 @@ int launch_copy(void* dst, const void* src, size_t bytes, void* stream) {
 int launch_upload(const UploadSpan* spans, uint32_t num_spans, uint64_t total_chunks, void* stream) {
   if (num_spans == 0 || total_chunks == 0) return 0;
   const unsigned blocks = (unsigned)std::min<uint64_t>((total_chunks + 255) / 256, 2048);
+  (void)hipGetLastError();  // see launch_trace
   hipLaunchKernelGGL(upload_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream, spans, num_spans, total_chunks);
   return (int)hipGetLastError();
 }
@@ -968,6 +970,9 @@ int launch_trace(const LaunchShape& shape, const TraceFrame* frames_dev, uint32_
   const uint32_t block = 64u * shape.waves;
   const dim3 grid((num_tiles + shape.waves - 1) / shape.waves);
   const size_t lds = shape.stack == SVO_STACK_LDS ? (size_t)stack_levels * block * sizeof(uint2) : 0;
+  // hipGetLastError reports the thread's last error from any earlier call:
+  // clear it so that the check below sees this launch only
+  (void)hipGetLastError();
   hipLaunchKernelGGL(fn, grid, dim3(block), lds, (hipStream_t)stream, frames_dev, num_frames, num_tiles);
   int e = (int)hipGetLastError();
   if (e != 0 || !shade) return e;
@@ -981,6 +986,7 @@ int launch_trace(const LaunchShape& shape, const TraceFrame* frames_dev, uint32_
 int launch_feedback_gather(const unsigned long long* fb_gid, const uint32_t* queue, uint32_t* count, uint32_t cap,
                            unsigned long long* out, void* stream) {
   const uint32_t blocks = std::max(1u, std::min(1024u, (cap + 255u) / 256u));
+  (void)hipGetLastError();  // see launch_trace
   hipLaunchKernelGGL(feedback_gather_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream, fb_gid, queue, count, cap,
                      out);
   hipLaunchKernelGGL(feedback_reset_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, count);
@@ -989,6 +995,7 @@ int launch_feedback_gather(const unsigned long long* fb_gid, const uint32_t* que
 
 int launch_ray_dirs(const svo_camera& cam, float* dirs_dev, void* stream) {
   const dim3 grid((cam.width + 63) / 64, cam.height), block(64);
+  (void)hipGetLastError();  // see launch_trace
   hipLaunchKernelGGL(ray_dir_kernel, grid, block, 0, (hipStream_t)stream, cam, dirs_dev);
   return (int)hipGetLastError();
 }

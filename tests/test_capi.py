@@ -28,6 +28,40 @@ def test_every_declared_function_is_exported():
     assert not unbound, f"declared but without ctypes signature: {unbound}"
 
 
+def test_ctypes_layouts_match_the_c_compiler(tmp_path):
+    """Every struct of include/svo_hip.h has the C compiler's size and field
+    offsets in the ctypes binding."""
+    import shutil
+    import subprocess
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("no gcc")
+    structs = [n for n in dir(_capi) if n.startswith("svo_") and isinstance(getattr(_capi, n), type)
+               and issubclass(getattr(_capi, n), ctypes.Structure)]
+    assert len(structs) >= 12
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "svo_hip.h"', "int main(void) {"]
+    for n in structs:
+        lines.append(f'  printf("{n} size %zu\\n", sizeof({n}));')
+        for f, _ in getattr(_capi, n)._fields_:
+            lines.append(f'  printf("{n} {f} %zu\\n", offsetof({n}, {f}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    subprocess.run([gcc, "-std=c99", "-I", os.path.join(root, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    want = {}
+    for line in filter(None, out):
+        name, field, value = line.split()
+        want[(name, field)] = int(value)
+    for n in structs:
+        cls = getattr(_capi, n)
+        assert ctypes.sizeof(cls) == want[(n, "size")], n
+        for f, _ in cls._fields_:
+            assert getattr(cls, f).offset == want[(n, f)], f"{n}.{f}"
+
+
 def test_struct_sizes_match_header_layout():
     assert ctypes.sizeof(_capi.svo_node) == 8
     assert ctypes.sizeof(_capi.svo_treelet_header) == 32

@@ -155,3 +155,92 @@ def test_plain_loop_is_one_view_per_launch(mms):
     assert stats["batches"] == len(reqs)
     assert stats["max_batch_views_seen"] == 1 and stats["max_inflight_seen"] == 1
     assert len(set(comp["batch"].tolist())) == len(reqs)
+
+
+def test_queue_bound_refuses_and_recovers(mms):
+    r = svo.Renderer(0)
+    srv = svo.Server(r, max_batch_views=1, max_inflight=1, max_waiting=3)
+    for mm in mms:
+        srv.add_scene(mm)
+    cams = [workloads.scene_camera("menger", 70 + i, 64, 64) for i in range(5)]
+    ids = [srv.submit(0, c) for c in cams[:3]]
+    with pytest.raises(svo.SvoError) as e:
+        srv.submit(0, cams[3])  # nothing has been launched yet: the queue holds 3
+    assert e.value.status == svo._capi.SVO_ERR_CAPACITY
+    done = []
+    while srv.pending:
+        done += srv.poll()
+    assert sorted(c["id"] for c in done) == sorted(ids)
+    assert all(c["status"] == svo._capi.SVO_REQ_DONE and c["batch_views"] == 1 for c in done)
+    srv.submit(0, cams[4])  # room again
+    while srv.pending:
+        srv.poll()
+
+
+def test_timeouts_drop_waiting_requests_unrendered(mms):
+    r = svo.Renderer(0)
+    srv = svo.Server(r, max_batch_views=1, max_inflight=1)
+    for mm in mms:
+        srv.add_scene(mm)
+    cam = workloads.scene_camera("terrain", 5, 256, 144)
+    keep = svo.Frame.alloc(256, 144)
+    first = srv.submit(1, cam, keep)  # no timeout
+    dropped = []
+    for i in range(4):
+        f = svo.Frame.alloc(64, 64)
+        f.rgba.fill(0xAB)
+        dropped.append((srv.submit(0, workloads.scene_camera("menger", 80 + i, 64, 64), f, timeout_us=1), f))
+    done = []
+    while srv.pending:
+        done += srv.poll()
+    status = {c["id"]: c for c in done}
+    assert status[first]["status"] == svo._capi.SVO_REQ_DONE
+    _same(keep.fetch(), _alone(r, mms[1], cam), "request that did not time out")
+    for rid, f in dropped:
+        c = status[rid]
+        assert c["status"] == svo._capi.SVO_REQ_EXPIRED and c["batch_views"] == 0
+        assert c["launch_s"] == c["done_s"] >= c["arrival_s"]
+        assert np.all(f.fetch()["rgba"] == 0xABABABAB), "an expired request must not be rendered"
+
+
+def test_replay_under_overload_reports_dropped_requests(mms):
+    r = svo.Renderer(0)
+    srv = svo.Server(r, max_batch_views=2, max_inflight=1, max_waiting=4)
+    for mm in mms:
+        srv.add_scene(mm)
+    reqs = _requests(80, 200000.0, seed=21)  # all but simultaneous
+    reqs = [(k, c, t, None, 300 if i % 2 else 0) for i, (k, c, t) in enumerate(reqs)]
+    stats, comp = srv.replay(reqs)
+    _say(f"overload stats {stats}")
+    st = comp["status"]
+    assert stats["requests"] == len(reqs)
+    assert stats["rejected"] == int((st == svo._capi.SVO_REQ_REJECTED).sum()) > 0
+    assert stats["expired"] == int((st == svo._capi.SVO_REQ_EXPIRED).sum())
+    rendered = st == svo._capi.SVO_REQ_DONE
+    assert rendered.sum() + stats["rejected"] + stats["expired"] == len(reqs)
+    # only requests with a timeout can expire
+    assert all(reqs[i][4] > 0 for i in np.flatnonzero(st == svo._capi.SVO_REQ_EXPIRED))
+    assert np.all(comp["batch_views"][~rendered] == 0) and np.all(comp["batch_views"][rendered] >= 1)
+
+
+def test_teardown_in_any_order(mms):
+    """A garbage-collection cycle may finalize a Renderer before the Server
+    or Batch made from it: their teardown must not read the renderer (it
+    used to, leaving an invalid-device HIP error behind)."""
+    import ctypes
+    import gc
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipPeekAtLastError.restype = ctypes.c_int
+    hip.hipGetLastError.restype = ctypes.c_int
+    hip.hipGetLastError()
+    for _ in range(3):
+        r = svo.Renderer(0)
+        srv = svo.Server(r, max_batch_views=2, max_inflight=1)
+        srv.add_scene(mms[0])
+        cam = workloads.scene_camera("menger", 3, 64, 64)
+        b = svo.Batch(r, [svo.View(mms[0], cam, svo.Frame.alloc(64, 64))], "treelet")
+        cycle = [r, srv, b]
+        cycle.append(cycle)  # the objects now live in a reference cycle
+        del r, srv, b, cycle
+        gc.collect()
+        assert hip.hipPeekAtLastError() == 0
