@@ -57,7 +57,10 @@ def profiled_traffic(shape):
     except OSError:
         return None
     key = f"| ({shape[0]}, {shape[1]}, {shape[2]}) |"
-    for line in text.splitlines():
+    section = text.find("## HBM traffic per launch shape")
+    if section < 0:
+        return None
+    for line in text[section:].splitlines():
         if line.startswith(key):
             m = re.findall(r"\|\s*([\d,]+)\s*", line[len(key) - 1:])
             if len(m) >= 3:
