@@ -17,7 +17,9 @@ Also reported in the same line:
 * ``ooc``: depth-13 terrain in 16384-node treelets (larger than the pool)
   behind an 8 GiB device pool, 300 1080p frames along a seeded fly-through;
   each frame runs the analyse pass, the feedback update (page-ins,
-  evictions), the upload and the render: frames/s and page-ins per frame;
+  evictions), the upload and the render: frames/s and page-ins per frame,
+  one stage after the other, and pipelined (frame i's render overlapping
+  frame i+1's analyse pass and feedback update);
 * ``serving``: Poisson request streams over four resident scenes (Menger
   depth 9, terrain depth 11, sphere cloud depth 10, terrain depth 12) in a
   64 GB HBM budget, served by the continuous-batching server and by the plain
@@ -233,7 +235,46 @@ def ooc_section(policy, frames, depth=13, treelet_bytes=131072, pool_bytes=8 << 
     log(f"ooc: {res['frames_per_s']} frames/s, {res['page_ins_per_frame']} page-ins/frame, "
         f"{res['upload_mib_per_frame']} MiB/frame uploaded, render {res['ms_render']} ms, "
         f"feedback {res['ms_feedback']} ms (analyse pass {res['ms_analyse_pass']} ms), upload {res['ms_upload']} ms")
+    resident_end = mm.info["resident_treelets"]
+    del view, mm
+    res["pipelined"] = _ooc_pipelined(sc, slots, cams, divide, policy, pages, resident_end)
+    log(f"ooc pipelined: {res['pipelined']}")
     return res
+
+
+def _ooc_pipelined(sc, slots, cams, divide, policy, seq_pages, seq_resident_end):
+    """The same fly-through with frame i's render overlapping the analyse
+    pass and feedback update of frame i+1 (second stream).  Both read the
+    residency frame i renders with, so every frame sees the same residency
+    as in the sequential loop; the upload of frame i+1 waits for render i."""
+    import numpy as np
+    import libsvo_amd as svo
+    from libsvo_amd import workloads
+    mm = svo.TreeletMemoryManagerHip(sc, slots * sc.info["treelet_bytes"], device=0)
+    rng = np.random.default_rng(workloads.FLIGHT_SEED)
+    jit = [tuple(float(v) for v in rng.uniform(-1.0, 1.0, 2)) for _ in cams]
+    frame = svo.Frame.alloc(cams[0].width, cams[0].height)
+    r = svo.Renderer(0)
+    s_render, s_analyse = svo.Stream(), svo.Stream()
+    view = [svo.View(mm, cams[0], frame)]
+    pages = []
+    wall0 = time.perf_counter()
+    mm.performAnalysePass(cams[0], divide, jit[0], policy, s_analyse)
+    st = mm.updateClientSideIncoreBuffer(s_analyse)
+    for i, cam in enumerate(cams):
+        s_render.synchronize()  # render i-1 has finished with the pools
+        mm.updateDeviceMemory(s_render)
+        view[0].camera = cam
+        r.render(view, policy, s_render)
+        pages.append(st["inserted"])
+        if i + 1 < len(cams):
+            mm.performAnalysePass(cams[i + 1], divide, jit[i + 1], policy, s_analyse)
+            st = mm.updateClientSideIncoreBuffer(s_analyse)
+    s_render.synchronize()
+    wall = time.perf_counter() - wall0
+    same = pages == list(seq_pages) and mm.info["resident_treelets"] == seq_resident_end
+    return {"frames_per_s": round(len(cams) / wall, 1), "ms_per_frame": round(wall * 1e3 / len(cams), 3),
+            "same_page_ins_as_sequential": bool(same)}
 
 
 def serving_section(cache, rates, seconds, policy):
