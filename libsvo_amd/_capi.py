@@ -59,7 +59,7 @@ class svo_mm_info(ctypes.Structure):
 
 
 class svo_upload_stats(ctypes.Structure):
-    _fields_ = [("slots_uploaded", u32), ("copy_calls", u32), ("bytes", u64)]
+    _fields_ = [("slots_uploaded", u32), ("copy_calls", u32), ("bytes", u64), ("bytes_moved", u64)]
 
 
 class svo_frame_out(ctypes.Structure):

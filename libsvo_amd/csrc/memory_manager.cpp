@@ -102,6 +102,10 @@ struct svo_mm {
   std::set<uint32_t> dirty;                          // _incoreSlotsToUpload
   uint32_t search_pos = 1;                           // freeIncorePosition's probe cursor
   uint32_t resident = 0;
+  // Per slot, the number of leading nodes that may be non-zero (a high-water
+  // mark; everything after it is zero) in the host buffers and on the
+  // device: fills and uploads skip the zero padding of partial treelets.
+  std::vector<uint32_t> extent, dev_extent;
   // HBM
   uint64_t* d_nodes = nullptr;
   uint32_t* d_colors = nullptr;
@@ -111,11 +115,11 @@ struct svo_mm {
   // analysis feedback tables (see TracePools) and their pinned readback
   unsigned long long* d_fb_slot = nullptr;
   unsigned long long* d_fb_gid = nullptr;
-  unsigned long long* d_fb_out = nullptr;
   uint32_t* d_fb_queue = nullptr;
   uint32_t* d_fb_count = nullptr;
   unsigned long long* h_fb_slot = nullptr;
-  unsigned long long* h_fb_out = nullptr;
+  unsigned long long* h_fb_gid = nullptr;  // the whole per-gid table, read back
+  uint32_t* h_zero = nullptr;               // a pinned 0 that resets the queue count
   uint32_t* h_fb_queue = nullptr;
   uint32_t* h_fb_count = nullptr;
   uint32_t fb_stamp = 1;
@@ -146,11 +150,11 @@ struct svo_mm {
     hipFree(d_slot_gid);
     hipFree(d_fb_slot);
     hipFree(d_fb_gid);
-    hipFree(d_fb_out);
     hipFree(d_fb_queue);
     hipFree(d_fb_count);
     if (h_fb_slot) hipHostFree(h_fb_slot);
-    if (h_fb_out) hipHostFree(h_fb_out);
+    if (h_fb_gid) hipHostFree(h_fb_gid);
+    if (h_zero) hipHostFree(h_zero);
     if (h_fb_queue) hipHostFree(h_fb_queue);
     if (h_fb_count) hipHostFree(h_fb_count);
     hipFree(d_spans);
@@ -163,15 +167,21 @@ struct svo_mm {
 
   // resetIncoreBuffer (src/TreeletMemoryManager.cpp:303-380), host part.
   void reset_host() {
-    std::fill(incore.begin(), incore.end(), svo_node{0, 0});
-    std::fill(normals.begin(), normals.end(), 0u);
-    std::fill(colors.begin(), colors.end(), 0u);
+    // zero what the slots' extents say may be non-zero (fresh buffers are
+    // zero already)
+    for (uint32_t sl = 0; sl < (uint32_t)extent.size(); ++sl) {
+      const uint64_t pos = (uint64_t)sl * N;
+      std::fill(incore.begin() + pos, incore.begin() + pos + extent[sl], svo_node{0, 0});
+      std::fill(normals.begin() + pos, normals.begin() + pos + extent[sl], 0u);
+      std::fill(colors.begin() + pos, colors.begin() + pos + extent[sl], 0u);
+    }
     slots.assign(num_slots, SlotInfo());
     treelet_slot.assign(scene->treelets.size(), -1);
     children_resident.assign(scene->treelets.size(), std::set<uint32_t>());
     free_slots.clear();
     for (uint32_t i = 1; i != num_slots + 1; ++i) free_slots.push_back(num_slots - i);  // top = slot 0
     dirty.clear();
+    extent.assign(num_slots, 0u);
     resident = 0;  // search_pos survives a reset, like the reference's static cursor
   }
 
@@ -383,14 +393,18 @@ struct svo_mm {
     if (op.kind == DataOp::kFill) {
       const uint64_t pos = (uint64_t)op.slot * N;
       const TreeletData& T = scene->treelets[op.gid];
-      std::copy(T.nodes.begin(), T.nodes.end(), incore.begin() + pos);
-      std::fill(incore.begin() + pos + T.h.num_nodes, incore.begin() + pos + N, svo_node{0, 0});
-      for (uint32_t i = 0; i < T.h.num_nodes; ++i) {
+      const uint32_t n = T.h.num_nodes, old = extent[op.slot];
+      std::copy(T.nodes.begin(), T.nodes.begin() + n, incore.begin() + pos);
+      for (uint32_t i = 0; i < n; ++i) {
         normals[pos + i] = T.attribs[2 * (size_t)i];
         colors[pos + i] = T.attribs[2 * (size_t)i + 1];
       }
-      std::fill(normals.begin() + pos + T.h.num_nodes, normals.begin() + pos + N, 0u);
-      std::fill(colors.begin() + pos + T.h.num_nodes, colors.begin() + pos + N, 0u);
+      if (old > n) {  // the previous occupant's nodes beyond this one's
+        std::fill(incore.begin() + pos + n, incore.begin() + pos + old, svo_node{0, 0});
+        std::fill(normals.begin() + pos + n, normals.begin() + pos + old, 0u);
+        std::fill(colors.begin() + pos + n, colors.begin() + pos + old, 0u);
+      }
+      extent[op.slot] = n;
       return;
     }
     const uint64_t leaf = (uint64_t)op.slot * N + h.parent_leaf_pos;
@@ -431,25 +445,36 @@ struct svo_mm {
   // Uploads runs of slots {first, count} (nodes, colours, normals) and the
   // slot table in one upload-kernel launch reading the pinned host buffers,
   // then waits for it: the host buffers may change again once this returns.
-  svo_status upload_runs(const std::vector<std::pair<uint32_t, uint32_t>>& runs, hipStream_t st, uint64_t* bytes) {
+  svo_status upload_runs(const std::vector<std::pair<uint32_t, uint32_t>>& runs, hipStream_t st, uint64_t* bytes,
+                         uint64_t* bytes_moved) {
     for (const auto& r : runs)
       if (bytes) *bytes += (uint64_t)r.second * N * 16;
     if (host_only()) return SVO_OK;
     uint32_t n = 0;
-    uint64_t chunks = 0;
+    uint64_t chunks = 0, moved = 0;
     auto add = [&](const void* src, void* dst, uint64_t b) {
       h_spans[n++] = UploadSpan{(uint64_t)(uintptr_t)src, (uint64_t)(uintptr_t)dst, b, chunks};
       chunks += (b + 15) / 16;
     };
-    if (runs.size() * 3 + 1 > span_cap) { set_error("upload: %zu runs exceed the span table", runs.size()); return SVO_ERR_STATE; }
+    uint64_t run_slots = 0;
+    for (const auto& r : runs) run_slots += r.second;
+    if (run_slots * 3 + 1 > span_cap) { set_error("upload: %llu slots exceed the span table", (unsigned long long)run_slots); return SVO_ERR_STATE; }
     for (const auto& r : runs) {
-      const size_t n0 = (size_t)r.first * N, nn = (size_t)r.second * N;
-      add(incore.data() + n0, d_nodes + n0, nn * 8);
-      add(colors.data() + n0, d_colors + n0, nn * 4);
-      add(normals.data() + n0, d_normals + n0, nn * 4);
+      for (uint32_t sl = r.first; sl < r.first + r.second; ++sl) {
+        // the host's non-zero part, and on the device whatever the previous
+        // contents left there
+        const size_t n0 = (size_t)sl * N, nn = std::max(extent[sl], dev_extent[sl]);
+        dev_extent[sl] = extent[sl];
+        if (nn == 0) continue;
+        add(incore.data() + n0, d_nodes + n0, nn * 8);
+        add(colors.data() + n0, d_colors + n0, nn * 4);
+        add(normals.data() + n0, d_normals + n0, nn * 4);
+        moved += nn * 16;
+      }
     }
     for (uint32_t s = 0; s < num_slots; ++s) h_slot_tab[s] = slots[s].gid;
     add(h_slot_tab, d_slot_gid, (uint64_t)num_slots * 4);
+    if (bytes_moved) *bytes_moved += moved + (uint64_t)num_slots * 4;
     HIP_TRY(hipMemcpyAsync(d_spans, h_spans, (size_t)n * sizeof(UploadSpan), hipMemcpyHostToDevice, st));
     const int e = launch_upload(d_spans, n, chunks, st);
     if (e != 0) { set_error("upload launch failed: %s", hipGetErrorString((hipError_t)e)); return SVO_ERR_HIP; }
@@ -460,7 +485,7 @@ struct svo_mm {
   // updateDeviceMemory (src/TreeletMemoryManagerCl.cpp:155-251): sorted
   // dirty slots are merged while (treelets+1)/(distance+1) >= 0.6*2.
   svo_status update_device(hipStream_t st, svo_upload_stats* stats) {
-    svo_upload_stats s{0, 0, 0};
+    svo_upload_stats s{0, 0, 0, 0};
     s.slots_uploaded = (uint32_t)dirty.size();
     if (!dirty.empty()) {
       const float limit = 0.6f * 2.0f;
@@ -483,7 +508,7 @@ struct svo_mm {
         ++s.copy_calls;
         runs.push_back({start, num_slots_r});
       }
-      svo_status r = upload_runs(runs, st, &s.bytes);
+      svo_status r = upload_runs(runs, st, &s.bytes, &s.bytes_moved);
       if (r != SVO_OK) return r;
       dirty.clear();
     }
@@ -540,9 +565,10 @@ extern "C" svo_status svo_mm_create(const svo_scene* scene, uint64_t incore_byte
   }
   const size_t ng = scene->treelets.size();
   if (hipMalloc(&mm->d_fb_slot, slots * 8) != hipSuccess || hipMalloc(&mm->d_fb_gid, ng * 8) != hipSuccess ||
-      hipMalloc(&mm->d_fb_out, ng * 8) != hipSuccess || hipMalloc(&mm->d_fb_queue, ng * 4) != hipSuccess ||
+      hipMalloc(&mm->d_fb_queue, ng * 4) != hipSuccess ||
       hipMalloc(&mm->d_fb_count, 4) != hipSuccess || hipHostMalloc(&mm->h_fb_slot, slots * 8, 0) != hipSuccess ||
-      hipHostMalloc(&mm->h_fb_out, ng * 8, 0) != hipSuccess || hipHostMalloc(&mm->h_fb_queue, ng * 4, 0) != hipSuccess ||
+      hipHostMalloc(&mm->h_fb_gid, ng * 8, 0) != hipSuccess || hipHostMalloc(&mm->h_fb_queue, ng * 4, 0) != hipSuccess ||
+      hipHostMalloc(&mm->h_zero, 4, 0) != hipSuccess ||
       hipHostMalloc(&mm->h_fb_count, 4, 0) != hipSuccess) {
     set_error("svo_mm_create: feedback tables for %zu treelets could not be allocated", ng);
     return SVO_ERR_NOMEM;
@@ -550,15 +576,21 @@ extern "C" svo_status svo_mm_create(const svo_scene* scene, uint64_t incore_byte
   HIP_TRY(hipMemset(mm->d_fb_slot, 0, slots * 8));
   HIP_TRY(hipMemset(mm->d_fb_gid, 0, ng * 8));
   HIP_TRY(hipMemset(mm->d_fb_count, 0, 4));
-  mm->span_cap = (uint32_t)(3 * slots + 1);
+  *mm->h_zero = 0u;
+  mm->span_cap = (uint32_t)(3 * slots + 1);  // three spans per slot, plus the slot table
   if (hipHostMalloc(&mm->h_spans, (size_t)mm->span_cap * sizeof(UploadSpan), 0) != hipSuccess ||
       hipMalloc(&mm->d_spans, (size_t)mm->span_cap * sizeof(UploadSpan)) != hipSuccess ||
       hipHostMalloc(&mm->h_slot_tab, slots * 4, 0) != hipSuccess) {
     set_error("svo_mm_create: upload tables for %llu slots could not be allocated", (unsigned long long)slots);
     return SVO_ERR_NOMEM;
   }
-  // initClBuffer: the whole incore buffer goes up once
-  r = mm->upload_runs({{0u, mm->num_slots}}, nullptr, nullptr);
+  // initClBuffer: the whole incore buffer goes up once; beyond the slots'
+  // extents it is zero, which the device gets from a memset
+  HIP_TRY(hipMemset(mm->d_nodes, 0, pool_nodes * 8));
+  HIP_TRY(hipMemset(mm->d_colors, 0, pool_nodes * 4));
+  HIP_TRY(hipMemset(mm->d_normals, 0, pool_nodes * 4));
+  mm->dev_extent.assign(mm->num_slots, 0u);
+  r = mm->upload_runs({{0u, mm->num_slots}}, nullptr, nullptr, nullptr);
   if (r != SVO_OK) return r;
   mm->dirty.clear();
   *out = mm.release();
@@ -837,21 +869,25 @@ extern "C" svo_status svo_mm_apply_records(svo_mm* mm, const uint32_t* slots, co
 namespace {
 
 // Reads this cycle's slot keys and asked gids back into mm->fb_seen /
-// fb_asked, then starts a new recording cycle.
+// fb_asked, then starts a new recording cycle.  Only copies (no kernel):
+// the slot keys, the queue of asked gids with its count and the per-gid
+// table come back whole, the queue count is reset from a pinned zero, and
+// the host picks each asked gid's key.  A copy does not wait for compute
+// units, so this overlaps with a render running on another stream.
 svo_status read_feedback_tables(svo_mm* mm, hipStream_t st) {
   HIP_TRY(hipSetDevice(mm->device));
   const uint32_t ng = (uint32_t)mm->scene->treelets.size();
+  // Let the recording launches finish first: copies queued behind a running
+  // kernel were measured to wait for a render on another stream as well
+  // (fly-through with overlapped renders: 614 -> 800 frames/s).
+  HIP_TRY(hipStreamSynchronize(st));
   HIP_TRY(hipMemcpyAsync(mm->h_fb_slot, mm->d_fb_slot, (size_t)mm->num_slots * 8, hipMemcpyDeviceToHost, st));
   HIP_TRY(hipMemcpyAsync(mm->h_fb_count, mm->d_fb_count, 4, hipMemcpyDeviceToHost, st));
-  int e = launch_feedback_gather(mm->d_fb_gid, mm->d_fb_queue, mm->d_fb_count, ng, mm->d_fb_out, st);
-  if (e != 0) { set_error("feedback gather launch failed: %s", hipGetErrorString((hipError_t)e)); return SVO_ERR_HIP; }
+  HIP_TRY(hipMemcpyAsync(mm->h_fb_queue, mm->d_fb_queue, (size_t)ng * 4, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(mm->h_fb_gid, mm->d_fb_gid, (size_t)ng * 8, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(mm->d_fb_count, mm->h_zero, 4, hipMemcpyHostToDevice, st));
   HIP_TRY(hipStreamSynchronize(st));
   const uint32_t n = std::min(*mm->h_fb_count, ng);
-  if (n) {
-    HIP_TRY(hipMemcpyAsync(mm->h_fb_queue, mm->d_fb_queue, (size_t)n * 4, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(mm->h_fb_out, mm->d_fb_out, (size_t)n * 8, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
-  }
   const uint32_t stamp = mm->fb_stamp;
   mm->fb_seen.clear();
   mm->fb_asked.clear();
@@ -859,7 +895,10 @@ svo_status read_feedback_tables(svo_mm* mm, hipStream_t st) {
     const unsigned long long k = mm->h_fb_slot[s];
     if ((uint32_t)(k >> 32) == stamp) mm->fb_seen.push_back({s, key_error(k)});
   }
-  for (uint32_t i = 0; i < n; ++i) mm->fb_asked.push_back({mm->h_fb_queue[i], key_error(mm->h_fb_out[i])});
+  for (uint32_t i = 0; i < n; ++i) {
+    const uint32_t g = mm->h_fb_queue[i];
+    if (g < ng) mm->fb_asked.push_back({g, key_error(mm->h_fb_gid[g])});
+  }
   ++mm->fb_stamp;  // records of this cycle are now stale
   return SVO_OK;
 }

@@ -81,11 +81,6 @@ svo_status stage_views(const svo_view* views, uint32_t n, int policy, const Laun
 LaunchShape renderer_shape(const svo_renderer* r);
 int renderer_device(const svo_renderer* r);
 
-// Feedback consumption: out[i] = fb_gid[queue[i]] for i < min(*count, cap),
-// then *count = 0 (a kernel, so that it may sit in a captured graph).
-int launch_feedback_gather(const unsigned long long* fb_gid, const uint32_t* queue, uint32_t* count, uint32_t cap,
-                           unsigned long long* out, void* stream);
-
 // Writes the normalised FP32 ray direction of every pixel of `cam`
 // (3 floats per pixel, row-major) -- used to pin the ray generation.
 int launch_ray_dirs(const svo_camera& cam, float* dirs_dev, void* stream);

@@ -855,16 +855,6 @@ __global__ __launch_bounds__(64) void shade_kernel(const TraceFrame* __restrict_
   reinterpret_cast<float4*>(F.out.shade)[pix] = shade_pixel(F.cam.view_mode, hit, d, F.pools);
 }
 
-__global__ void feedback_gather_kernel(const unsigned long long* __restrict__ fb_gid, const uint32_t* __restrict__ queue,
-                                       uint32_t* count, uint32_t cap, unsigned long long* __restrict__ out) {
-  const uint32_t n = min(*count, cap);
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) out[i] = fb_gid[queue[i]];
-}
-
-__global__ void feedback_reset_kernel(uint32_t* count) {
-  if (threadIdx.x == 0) *count = 0u;
-}
-
 __global__ void ray_dir_kernel(svo_camera cam, float* dirs) {
   const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t y = blockIdx.y;
@@ -980,16 +970,6 @@ int launch_trace(const LaunchShape& shape, const TraceFrame* frames_dev, uint32_
   if (shape.tile_w == 8) hipLaunchKernelGGL(shade_kernel<8>, sgrid, sblock, 0, (hipStream_t)stream, frames_dev, num_frames, num_tiles);
   else if (shape.tile_w == 32) hipLaunchKernelGGL(shade_kernel<32>, sgrid, sblock, 0, (hipStream_t)stream, frames_dev, num_frames, num_tiles);
   else hipLaunchKernelGGL(shade_kernel<16>, sgrid, sblock, 0, (hipStream_t)stream, frames_dev, num_frames, num_tiles);
-  return (int)hipGetLastError();
-}
-
-int launch_feedback_gather(const unsigned long long* fb_gid, const uint32_t* queue, uint32_t* count, uint32_t cap,
-                           unsigned long long* out, void* stream) {
-  const uint32_t blocks = std::max(1u, std::min(1024u, (cap + 255u) / 256u));
-  (void)hipGetLastError();  // see launch_trace
-  hipLaunchKernelGGL(feedback_gather_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream, fb_gid, queue, count, cap,
-                     out);
-  hipLaunchKernelGGL(feedback_reset_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, count);
   return (int)hipGetLastError();
 }
 
