@@ -252,7 +252,11 @@ typedef struct svo_mm_info {
 typedef struct svo_upload_stats {
   uint32_t slots_uploaded;    /* zz_memory_manager-cl_slots_to_upload   */
   uint32_t copy_calls;        /* zz_memory_manager-num_copy_calls       */
-  uint64_t bytes;
+  uint64_t bytes;             /* the merged ranges' bytes (what the
+                                 reference's enqueueWrite calls move)   */
+  uint64_t bytes_moved;       /* bytes actually sent: within each range
+                                 only the part of a slot that is, or on
+                                 the device was, not zero padding       */
 } svo_upload_stats;
 
 /* TreeletMemoryManagerCl(path, incoreBufferSizeInByte, ctx) +
@@ -281,8 +285,9 @@ svo_status svo_mm_remove(svo_mm* mm, uint32_t gid);
 svo_status svo_mm_make_resident_all(svo_mm* mm);
 
 /* updateDeviceMemory(): dirty slots merged into ranges with the reference
- * ratio rule (src/TreeletMemoryManagerCl.cpp:155-251) and copied to HBM
- * with hipMemcpyAsync on `stream`, followed by a stream synchronize. */
+ * ratio rule (src/TreeletMemoryManagerCl.cpp:155-251) and copied to HBM in
+ * one launch on `stream` that reads the pinned host buffers, followed by a
+ * stream synchronize.  The device ends equal to the host incore buffer. */
 svo_status svo_mm_update_device(svo_mm* mm, void* stream,
                                 svo_upload_stats* stats);
 
