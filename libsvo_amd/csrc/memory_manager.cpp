@@ -264,21 +264,52 @@ struct svo_mm {
   // The visible sets (performAnalysePass's host half): `old` gets every
   // seen treelet and its ancestors with the largest error, `asked` is sorted
   // and capped.  Returns the number of seen slots that hold a treelet.
+  // Scratch of visible_sets: the largest error per gid, valid where
+  // vis_mark[gid] == vis_epoch.
+  mutable std::vector<float> vis_err;
+  mutable std::vector<uint32_t> vis_mark;
+  mutable uint32_t vis_epoch = 0;
+
   uint32_t visible_sets(const std::vector<std::pair<uint32_t, float>>& seen,
-                        std::vector<std::pair<uint32_t, float>>* asked, std::map<uint32_t, float>* old) const {
+                        std::vector<std::pair<uint32_t, float>>* asked,
+                        std::vector<std::pair<uint32_t, float>>* old) const {
+    const size_t ng = scene->treelets.size();
+    if (vis_mark.size() != ng) {
+      vis_err.assign(ng, 0.0f);
+      vis_mark.assign(ng, 0u);
+      vis_epoch = 0;
+    }
+    if (++vis_epoch == 0) {  // wrapped: start the marks afresh
+      std::fill(vis_mark.begin(), vis_mark.end(), 0u);
+      vis_epoch = 1;
+    }
+    std::vector<uint32_t> touched;
+    bool root = false;
     uint32_t visible_slots = 0;
     for (const auto& e : seen) {
       if (e.first >= num_slots || slots[e.first].gid < 0) continue;
       ++visible_slots;
-      uint32_t g = (uint32_t)slots[e.first].gid;
-      while (g != 0) {
-        auto it = old->find(g);
-        if (it == old->end()) (*old)[g] = e.second;
-        else if (e.second > it->second) it->second = e.second;
-        g = scene->treelets[g].h.parent_gid;
+      root = true;  // the root with error 99 (RenderPassAnalyse.cpp)
+      // the treelet and its ancestors keep the largest error seen below
+      // them; an ancestor that already holds at least this error passes it
+      // on to its own ancestors already, so the walk stops there
+      for (uint32_t g = (uint32_t)slots[e.first].gid; g != 0; g = scene->treelets[g].h.parent_gid) {
+        if (vis_mark[g] != vis_epoch) {
+          vis_mark[g] = vis_epoch;
+          vis_err[g] = e.second;
+          touched.push_back(g);
+        } else if (e.second > vis_err[g]) {
+          vis_err[g] = e.second;
+        } else if (vis_err[g] >= e.second) {
+          break;
+        }
       }
-      (*old)[0] = 99.0f;
     }
+    std::sort(touched.begin(), touched.end());
+    old->clear();
+    old->reserve(touched.size() + 1);
+    if (root) old->push_back({0u, 99.0f});
+    for (uint32_t g : touched) old->push_back({g, vis_err[g]});
     std::stable_sort(asked->begin(), asked->end(), [](const std::pair<uint32_t, float>& a, const std::pair<uint32_t, float>& b) {
       return a.first < b.first;
     });
@@ -293,7 +324,7 @@ struct svo_mm {
   // The sets one analysis cycle yields, computed against the slot table of
   // the moment (as performAnalysePass does), applied later.
   struct FeedbackSets {
-    std::map<uint32_t, float> old;
+    std::vector<std::pair<uint32_t, float>> old;  // ascending gid
     std::vector<std::pair<uint32_t, float>> asked;  // sorted, capped
     uint32_t visible_slots = 0;
     uint32_t requests = 0;                          // before the cap
