@@ -106,6 +106,11 @@ struct svo_mm {
   // mark; everything after it is zero) in the host buffers and on the
   // device: fills and uploads skip the zero padding of partial treelets.
   std::vector<uint32_t> extent, dev_extent;
+  // What the next upload must send: slots whose content was replaced (a
+  // fill or a reset) go up within their extents; otherwise only the nodes
+  // a stitch or unstitch changed (global node indices).
+  std::vector<uint8_t> refilled;
+  std::vector<uint64_t> patched;
   // HBM
   uint64_t* d_nodes = nullptr;
   uint32_t* d_colors = nullptr;
@@ -182,6 +187,8 @@ struct svo_mm {
     for (uint32_t i = 1; i != num_slots + 1; ++i) free_slots.push_back(num_slots - i);  // top = slot 0
     dirty.clear();
     extent.assign(num_slots, 0u);
+    refilled.assign(num_slots, 1u);  // everything goes up after a reset
+    patched.clear();
     resident = 0;  // search_pos survives a reset, like the reference's static cursor
   }
 
@@ -436,6 +443,7 @@ struct svo_mm {
         std::fill(colors.begin() + pos + n, colors.begin() + pos + old, 0u);
       }
       extent[op.slot] = n;
+      refilled[op.slot] = 1u;
       return;
     }
     const uint64_t leaf = (uint64_t)op.slot * N + h.parent_leaf_pos;
@@ -451,6 +459,10 @@ struct svo_mm {
     } else {
       incore[leaf] = scene->treelets[h.parent_gid].nodes[h.parent_leaf_pos];  // pristine leaf
       incore[leaf_parent].masks |= 1u << (kLeafShift + h.parent_leaf_idx);
+    }
+    if (!host_only()) {  // a host-only handle uploads nothing
+      patched.push_back(leaf);
+      patched.push_back(leaf_parent);
     }
   }
 
@@ -489,11 +501,30 @@ struct svo_mm {
     };
     uint64_t run_slots = 0;
     for (const auto& r : runs) run_slots += r.second;
-    if (run_slots * 3 + 1 > span_cap) { set_error("upload: %llu slots exceed the span table", (unsigned long long)run_slots); return SVO_ERR_STATE; }
+    if (run_slots * 3 + patched.size() + 1 > span_cap) {
+      // grow the span tables (pinned and device) for this upload
+      const uint64_t cap = run_slots * 3 + patched.size() + 1;
+      if (cap > 0xFFFFFFFFull) { set_error("upload: %llu spans", (unsigned long long)cap); return SVO_ERR_STATE; }
+      HIP_TRY(hipStreamSynchronize(st));
+      HIP_TRY(hipFree(d_spans));
+      HIP_TRY(hipHostFree(h_spans));
+      d_spans = nullptr;
+      h_spans = nullptr;
+      span_cap = 0;
+      if (hipHostMalloc(&h_spans, cap * sizeof(UploadSpan), 0) != hipSuccess ||
+          hipMalloc(&d_spans, cap * sizeof(UploadSpan)) != hipSuccess) {
+        set_error("upload: span tables of %llu entries could not be allocated", (unsigned long long)cap);
+        return SVO_ERR_NOMEM;
+      }
+      span_cap = (uint32_t)cap;
+    }
+    // The merged ranges name what the reference would copy; what moves is
+    // the refilled slots within their extents (the host's non-zero part and
+    // whatever the device still holds beyond it) and the stitched nodes of
+    // slots that were not refilled.  The device ends equal to the host.
     for (const auto& r : runs) {
       for (uint32_t sl = r.first; sl < r.first + r.second; ++sl) {
-        // the host's non-zero part, and on the device whatever the previous
-        // contents left there
+        if (!refilled[sl]) continue;
         const size_t n0 = (size_t)sl * N, nn = std::max(extent[sl], dev_extent[sl]);
         dev_extent[sl] = extent[sl];
         if (nn == 0) continue;
@@ -503,6 +534,13 @@ struct svo_mm {
         moved += nn * 16;
       }
     }
+    std::sort(patched.begin(), patched.end());
+    patched.erase(std::unique(patched.begin(), patched.end()), patched.end());
+    for (const uint64_t node : patched) {
+      if (refilled[node / N]) continue;
+      add(incore.data() + node, d_nodes + node, 8);
+      moved += 8;
+    }
     for (uint32_t s = 0; s < num_slots; ++s) h_slot_tab[s] = slots[s].gid;
     add(h_slot_tab, d_slot_gid, (uint64_t)num_slots * 4);
     if (bytes_moved) *bytes_moved += moved + (uint64_t)num_slots * 4;
@@ -510,6 +548,8 @@ struct svo_mm {
     const int e = launch_upload(d_spans, n, chunks, st);
     if (e != 0) { set_error("upload launch failed: %s", hipGetErrorString((hipError_t)e)); return SVO_ERR_HIP; }
     HIP_TRY(hipStreamSynchronize(st));
+    for (const auto& r : runs) std::fill(refilled.begin() + r.first, refilled.begin() + r.first + r.second, 0u);
+    patched.clear();
     return SVO_OK;
   }
 
