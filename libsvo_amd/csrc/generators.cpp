@@ -137,7 +137,8 @@ extern "C" svo_status svo_scene_build_terrain(uint32_t depth, uint64_t seed, uin
   const uint32_t n = 1u << depth;
   // Column fill heights: voxel j is filled iff its centre (j+0.5)/n < h.
   std::vector<uint32_t> H((size_t)n * n);
-  for (uint32_t i = 0; i < n; ++i)
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < (int64_t)n; ++i)
     for (uint32_t k = 0; k < n; ++k) {
       const double h = terrain_h(seed, (i + 0.5) / n, (k + 0.5) / n);
       double c = std::ceil(h * n - 0.5);
@@ -146,30 +147,52 @@ extern "C" svo_status svo_scene_build_terrain(uint32_t depth, uint64_t seed, uin
     }
   // Shell of each column: j = 0 (bottom face) and [lo, H-1], where lo is the
   // lowest voxel whose side faces an empty neighbour column (0 on the cube's
-  // side walls).
-  std::vector<uint64_t> codes;
-  for (uint32_t i = 0; i < n; ++i)
-    for (uint32_t k = 0; k < n; ++k) {
-      const uint32_t h = H[(size_t)i * n + k];
-      if (h == 0) continue;
-      uint32_t lo = h - 1;
-      if (i == 0 || k == 0 || i == n - 1 || k == n - 1) lo = 0;
-      else {
-        lo = std::min(lo, H[(size_t)(i - 1) * n + k]);
-        lo = std::min(lo, H[(size_t)(i + 1) * n + k]);
-        lo = std::min(lo, H[(size_t)i * n + k - 1]);
-        lo = std::min(lo, H[(size_t)i * n + k + 1]);
-      }
-      if (lo > 0) codes.push_back(morton_encode(i, 0, k, depth));
-      for (uint32_t j = lo; j < h; ++j) codes.push_back(morton_encode(i, j, k, depth));
+  // side walls).  Rows are counted, then filled, in parallel.
+  auto shell = [&](uint32_t i, uint32_t k, uint32_t* lo_out) {
+    const uint32_t h = H[(size_t)i * n + k];
+    if (h == 0) return false;
+    uint32_t lo = h - 1;
+    if (i == 0 || k == 0 || i == n - 1 || k == n - 1) lo = 0;
+    else {
+      lo = std::min(lo, H[(size_t)(i - 1) * n + k]);
+      lo = std::min(lo, H[(size_t)(i + 1) * n + k]);
+      lo = std::min(lo, H[(size_t)i * n + k - 1]);
+      lo = std::min(lo, H[(size_t)i * n + k + 1]);
     }
+    *lo_out = lo;
+    return true;
+  };
+  std::vector<uint64_t> row_start((size_t)n + 1, 0);
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int64_t i = 0; i < (int64_t)n; ++i) {
+    uint64_t c = 0;
+    for (uint32_t k = 0; k < n; ++k) {
+      uint32_t lo;
+      if (!shell((uint32_t)i, k, &lo)) continue;
+      c += (lo > 0 ? 1 : 0) + (H[(size_t)i * n + k] - lo);
+    }
+    row_start[(size_t)i + 1] = c;
+  }
+  for (uint32_t i = 0; i < n; ++i) row_start[i + 1] += row_start[i];
+  std::vector<uint64_t> codes(row_start[n]);
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int64_t i = 0; i < (int64_t)n; ++i) {
+    uint64_t q = row_start[(size_t)i];
+    for (uint32_t k = 0; k < n; ++k) {
+      uint32_t lo;
+      if (!shell((uint32_t)i, k, &lo)) continue;
+      if (lo > 0) codes[q++] = morton_encode((uint32_t)i, 0, k, depth);
+      for (uint32_t j = lo; j < H[(size_t)i * n + k]; ++j) codes[q++] = morton_encode((uint32_t)i, j, k, depth);
+    }
+  }
   radix_sort_u64(codes, 3 * depth);
   std::vector<uint32_t> attr(2 * codes.size());
-  for (size_t q = 0; q < codes.size(); ++q) {
+#pragma omp parallel for schedule(static)
+  for (int64_t q = 0; q < (int64_t)codes.size(); ++q) {
     uint32_t i, j, k;
-    morton_decode(codes[q], depth, &i, &j, &k);
-    attr[2 * q] = leaf_normal_up();
-    attr[2 * q + 1] = terrain_color(seed, i, j, k, depth);
+    morton_decode(codes[(size_t)q], depth, &i, &j, &k);
+    attr[2 * (size_t)q] = leaf_normal_up();
+    attr[2 * (size_t)q + 1] = terrain_color(seed, i, j, k, depth);
   }
   return finish(depth, codes, attr, treelet_bytes, flags, out);
 }

@@ -33,6 +33,8 @@
 #include <deque>
 #include <memory>
 
+#include <omp.h>
+
 #include "svo_internal.h"
 
 namespace svo_impl {
@@ -82,13 +84,32 @@ void radix_sort_u64(std::vector<uint64_t>& keys, uint32_t bits) {
   if (n < 2) return;
   std::vector<uint64_t> tmp(n);
   const uint32_t kRadix = 11, kBuckets = 1u << kRadix;
-  std::vector<size_t> count(kBuckets);
+  // LSD passes; each thread histograms and scatters its own contiguous chunk,
+  // the chunks' offsets per bucket following in thread order (stable)
+  const int nt = n < (1u << 16) ? 1 : std::max(1, omp_get_max_threads());
+  std::vector<size_t> count((size_t)nt * kBuckets);
   for (uint32_t shift = 0; shift < bits; shift += kRadix) {
-    std::fill(count.begin(), count.end(), 0);
-    for (size_t i = 0; i < n; ++i) ++count[(keys[i] >> shift) & (kBuckets - 1)];
-    size_t sum = 0;
-    for (uint32_t b = 0; b < kBuckets; ++b) { size_t c = count[b]; count[b] = sum; sum += c; }
-    for (size_t i = 0; i < n; ++i) tmp[count[(keys[i] >> shift) & (kBuckets - 1)]++] = keys[i];
+#pragma omp parallel num_threads(nt)
+    {
+      const int t = omp_get_thread_num();
+      const size_t lo = n * t / nt, hi = n * (t + 1) / nt;
+      size_t* c = &count[(size_t)t * kBuckets];
+      std::fill(c, c + kBuckets, 0);
+      for (size_t i = lo; i < hi; ++i) ++c[(keys[i] >> shift) & (kBuckets - 1)];
+#pragma omp barrier
+#pragma omp single
+      {
+        size_t sum = 0;
+        for (uint32_t b = 0; b < kBuckets; ++b)
+          for (int u = 0; u < nt; ++u) {
+            size_t& x = count[(size_t)u * kBuckets + b];
+            const size_t v = x;
+            x = sum;
+            sum += v;
+          }
+      }
+      for (size_t i = lo; i < hi; ++i) tmp[c[(keys[i] >> shift) & (kBuckets - 1)]++] = keys[i];
+    }
     keys.swap(tmp);
   }
 }

@@ -212,3 +212,25 @@ def test_tiny_tree_in_one_treelet(tmp_path):
     sc.write(str(tmp_path / "tiny"))
     back = svo.Scene.open(str(tmp_path / "tiny"))
     assert back.info == sc.info
+
+
+# Hashes of the depth-9 terrain (seed 3, 4096-byte treelets) as written by the
+# single-threaded builder; the generator and the radix sort split their work
+# over threads from about 65k leaves on, and the bytes must not change.
+TERRAIN_D9_SVO = "ba8c00bcb445ea4b8bf86aef309485cf51ac97391b996dd96a1af543ecb1a0a6"
+TERRAIN_D9_IA = "da105f4afa083bed3d2be66a503f35d263d4975d0f04540af42e447834e44cca"
+
+
+@pytest.mark.parametrize("threads", [1, 3, 8])
+def test_terrain_bytes_do_not_depend_on_thread_count(tmp_path, threads):
+    import hashlib
+    import subprocess
+    import sys
+    import os
+    base = str(tmp_path / "t")
+    code = ("import libsvo_amd as svo, sys; svo.Scene.terrain(9, 3, 4096).write(sys.argv[1])")
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    subprocess.run([sys.executable, "-c", code, base], check=True, env=env, cwd=root, timeout=120)
+    assert hashlib.sha256(open(base + ".svo", "rb").read()).hexdigest() == TERRAIN_D9_SVO
+    assert hashlib.sha256(open(base + ".ia", "rb").read()).hexdigest() == TERRAIN_D9_IA
