@@ -107,19 +107,34 @@ extern "C" svo_status svo_scene_build_menger(uint32_t level, uint32_t depth, uin
     if (i < 0 || j < 0 || k < 0 || i >= n || j >= n || k >= n) return false;
     return menger_inside((uint32_t)i, (uint32_t)j, (uint32_t)k, level, depth);
   };
+  // The Morton range is cut into ordered parts filled in parallel and joined
+  // in order, so the codes stay sorted.
+  const uint64_t total = 1ull << (3 * depth);
+  const uint64_t parts = std::min<uint64_t>(total, 1024);
+  std::vector<std::vector<uint64_t>> pc(parts);
+  std::vector<std::vector<uint32_t>> pa(parts);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int64_t part = 0; part < (int64_t)parts; ++part) {
+    const uint64_t c0 = total * part / parts, c1 = total * (part + 1) / parts;
+    for (uint64_t c = c0; c < c1; ++c) {
+      uint32_t i, j, k;
+      morton_decode(c, depth, &i, &j, &k);
+      if (!inside(i, j, k)) continue;
+      const bool shell = !inside((int64_t)i - 1, j, k) || !inside((int64_t)i + 1, j, k) || !inside(i, (int64_t)j - 1, k) ||
+                         !inside(i, (int64_t)j + 1, k) || !inside(i, j, (int64_t)k - 1) || !inside(i, j, (int64_t)k + 1);
+      if (!shell) continue;
+      pc[part].push_back(c);
+      pa[part].push_back(leaf_normal_up());
+      pa[part].push_back(hash_color(seed, i, j, k));
+    }
+  }
   std::vector<uint64_t> codes;
   std::vector<uint32_t> attr;
-  const uint64_t total = 1ull << (3 * depth);
-  for (uint64_t c = 0; c < total; ++c) {  // Morton order -> already sorted
-    uint32_t i, j, k;
-    morton_decode(c, depth, &i, &j, &k);
-    if (!inside(i, j, k)) continue;
-    const bool shell = !inside((int64_t)i - 1, j, k) || !inside((int64_t)i + 1, j, k) || !inside(i, (int64_t)j - 1, k) ||
-                       !inside(i, (int64_t)j + 1, k) || !inside(i, j, (int64_t)k - 1) || !inside(i, j, (int64_t)k + 1);
-    if (!shell) continue;
-    codes.push_back(c);
-    attr.push_back(leaf_normal_up());
-    attr.push_back(hash_color(seed, i, j, k));
+  for (uint64_t part = 0; part < parts; ++part) {
+    codes.insert(codes.end(), pc[part].begin(), pc[part].end());
+    attr.insert(attr.end(), pa[part].begin(), pa[part].end());
+    std::vector<uint64_t>().swap(pc[part]);
+    std::vector<uint32_t>().swap(pa[part]);
   }
   return finish(depth, codes, attr, treelet_bytes, flags, out);
 }

@@ -59,24 +59,39 @@ namespace svo_impl {
 
 // ------------------------------------------------------------ morton
 
+// Bit b of i, j, k goes to bit 3b+2, 3b+1, 3b of the code (bits >= depth
+// dropped).  The spread and its inverse move bit groups of halving width
+// with shifts and masks (21 bits per axis).
+static inline uint64_t spread3(uint64_t x) {
+  x &= 0x1FFFFFull;
+  x = (x | x << 32) & 0x001F00000000FFFFull;
+  x = (x | x << 16) & 0x001F0000FF0000FFull;
+  x = (x | x << 8) & 0x100F00F00F00F00Full;
+  x = (x | x << 4) & 0x10C30C30C30C30C3ull;
+  x = (x | x << 2) & 0x1249249249249249ull;
+  return x;
+}
+
+static inline uint32_t compact3(uint64_t x) {
+  x &= 0x1249249249249249ull;
+  x = (x | x >> 2) & 0x10C30C30C30C30C3ull;
+  x = (x | x >> 4) & 0x100F00F00F00F00Full;
+  x = (x | x >> 8) & 0x001F0000FF0000FFull;
+  x = (x | x >> 16) & 0x001F00000000FFFFull;
+  x = (x | x >> 32) & 0x1FFFFFull;
+  return (uint32_t)x;
+}
+
 uint64_t morton_encode(uint32_t i, uint32_t j, uint32_t k, uint32_t depth) {
-  uint64_t c = 0;
-  for (uint32_t b = 0; b < depth; ++b) {
-    c |= (uint64_t)((i >> b) & 1u) << (3 * b + 2);
-    c |= (uint64_t)((j >> b) & 1u) << (3 * b + 1);
-    c |= (uint64_t)((k >> b) & 1u) << (3 * b);
-  }
-  return c;
+  const uint32_t m = depth >= 21 ? 0x1FFFFFu : (1u << depth) - 1u;
+  return spread3(i & m) << 2 | spread3(j & m) << 1 | spread3(k & m);
 }
 
 void morton_decode(uint64_t c, uint32_t depth, uint32_t* i, uint32_t* j, uint32_t* k) {
-  uint32_t x = 0, y = 0, z = 0;
-  for (uint32_t b = 0; b < depth; ++b) {
-    x |= (uint32_t)((c >> (3 * b + 2)) & 1u) << b;
-    y |= (uint32_t)((c >> (3 * b + 1)) & 1u) << b;
-    z |= (uint32_t)((c >> (3 * b)) & 1u) << b;
-  }
-  *i = x; *j = y; *k = z;
+  const uint32_t m = depth >= 21 ? 0x1FFFFFu : (1u << depth) - 1u;
+  *i = compact3(c >> 2) & m;
+  *j = compact3(c >> 1) & m;
+  *k = compact3(c) & m;
 }
 
 void radix_sort_u64(std::vector<uint64_t>& keys, uint32_t bits) {
@@ -202,23 +217,48 @@ svo_status build_scene(uint32_t depth, std::vector<uint64_t>& codes,
   cur.swap(codes);
   uint64_t tree_nodes = cur.size();
   const uint64_t leaves = cur.size();
+  // Each level is grouped in parallel: thread chunks start at a parent
+  // boundary, count their parents, then write them at their prefix offsets.
+  const int nt = std::max(1, omp_get_max_threads());
+  std::vector<size_t> chunk(nt + 1), nparents(nt + 1);
   for (int d = (int)depth - 1; d >= 0; --d) {
     Level& P = lv[d];
     const Level& C = lv[d + 1];
-    std::vector<uint64_t> parent_codes;
-    parent_codes.reserve(cur.size() / 2 + 1);
-    for (size_t i = 0; i < cur.size(); ++i) {
-      uint64_t p = cur[i] >> 3;
-      if (parent_codes.empty() || parent_codes.back() != p) {
-        parent_codes.push_back(p);
-        P.mask.push_back(0);
-        P.first.push_back((uint32_t)i);
-      }
-      P.mask.back() |= (uint8_t)(1u << (cur[i] & 7u));
+    const size_t nc = cur.size();
+    const int use = nc < (1u << 16) ? 1 : nt;
+    for (int t = 0; t <= use; ++t) {
+      size_t b = nc * t / use;
+      while (b > 0 && b < nc && (cur[b] >> 3) == (cur[b - 1] >> 3)) ++b;
+      chunk[t] = b;
     }
-    const size_t np = parent_codes.size();
+    nparents[0] = 0;
+#pragma omp parallel for num_threads(use) schedule(static, 1)
+    for (int t = 0; t < use; ++t) {
+      size_t c = 0;
+      for (size_t i = chunk[t]; i < chunk[t + 1]; ++i) c += (i == chunk[t] || (cur[i] >> 3) != (cur[i - 1] >> 3));
+      nparents[t + 1] = c;
+    }
+    for (int t = 0; t < use; ++t) nparents[t + 1] += nparents[t];
+    const size_t np = nparents[use];
+    std::vector<uint64_t> parent_codes(np);
+    P.mask.assign(np, 0);
+    P.first.resize(np);
     P.attr.resize(2 * np);
-    for (size_t p = 0; p < np; ++p) {
+#pragma omp parallel for num_threads(use) schedule(static, 1)
+    for (int t = 0; t < use; ++t) {
+      size_t q = nparents[t];
+      for (size_t i = chunk[t]; i < chunk[t + 1]; ++i) {
+        const uint64_t pc = cur[i] >> 3;
+        if (i == chunk[t] || pc != (cur[i - 1] >> 3)) {
+          parent_codes[q] = pc;
+          P.first[q] = (uint32_t)i;
+          ++q;
+        }
+        P.mask[q - 1] |= (uint8_t)(1u << (cur[i] & 7u));
+      }
+    }
+#pragma omp parallel for schedule(static) if (np >= 4096)
+    for (int64_t p = 0; p < (int64_t)np; ++p) {
       uint32_t cnt = (uint32_t)__builtin_popcount(P.mask[p]);
       average_attribs(&C.attr[2 * (size_t)P.first[p]], cnt, &P.attr[2 * p]);
     }
@@ -261,7 +301,13 @@ svo_status build_scene(uint32_t depth, std::vector<uint64_t>& codes,
       w[d].assign(np, 0);
       cut[d].assign(np, 0);
       const bool child_even = ((d + 1) % 2) == 0 && d + 1 >= 2;
-      for (size_t p = 0; p < np; ++p) {
+      // A node reads and writes only its own children and grandchildren, so
+      // the nodes of a level are independent; the lowest failing node names
+      // the error.
+      int64_t bad = -1;
+      uint64_t bad_sum = 0;
+#pragma omp parallel for schedule(dynamic, 1024) if (np >= 4096)
+      for (int64_t p = 0; p < (int64_t)np; ++p) {
         const uint32_t f = P.first[p];
         const uint32_t nc = (uint32_t)__builtin_popcount(P.mask[p]);
         uint64_t sum = 1;
@@ -292,10 +338,15 @@ svo_status build_scene(uint32_t depth, std::vector<uint64_t>& codes,
             sum -= dec;
             if (cand[k].level == (uint32_t)d + 2) w[d + 1][cand[k].parent_slot] -= (uint32_t)dec;
           }
-          if (sum > N) { set_error("dense packing: node at level %d keeps %llu > %u nodes", d, (unsigned long long)sum, N); return SVO_ERR_ARG; }
+          if (sum > N) {
+#pragma omp critical(svo_dense_cut_error)
+            if (bad < 0 || p < bad) { bad = p; bad_sum = sum; }
+            continue;
+          }
         }
         w[d][p] = (uint32_t)sum;
       }
+      if (bad >= 0) { set_error("dense packing: node at level %d keeps %llu > %u nodes", d, (unsigned long long)bad_sum, N); return SVO_ERR_ARG; }
       if (d + 2 <= (int)depth) std::vector<uint32_t>().swap(w[d + 2]);
     }
   }
@@ -307,27 +358,31 @@ svo_status build_scene(uint32_t depth, std::vector<uint64_t>& codes,
   sc->tree_nodes = tree_nodes;
   sc->tree_leaves = leaves;
 
-  std::vector<std::vector<Incomplete>> incomplete;  // per treelet, freed when expanded
-
-  auto build_treelet = [&](uint32_t gid, uint32_t root_level, uint32_t root_index,
-                           uint32_t parent_gid, uint32_t leaf_pos, uint32_t leaf_idx,
-                           uint32_t leaf_parent_pos) {
-    TreeletData T;
+  // Treelet contents depend only on their root node and the cuts, so each
+  // generation (the treelets anchored in the previous one) is expanded in
+  // parallel.  Gids follow the reference's FIFO order over treelets: the
+  // sub-treelets of treelet t are numbered after those of t-1, in the order
+  // of t's anchor leaves.
+  struct Root { uint32_t level, index, parent_gid, leaf_pos, leaf_idx, leaf_parent_pos; };
+  auto build_treelet = [&](uint32_t gid, const Root& r, std::vector<Incomplete>& inc) {
+    TreeletData& T = sc->treelets[gid];
     T.h.mem_size = tb;
     T.h.gid = gid;
-    T.h.parent_gid = parent_gid;
-    T.h.parent_leaf_pos = leaf_pos;
-    T.h.parent_leaf_idx = leaf_idx;
-    T.h.parent_leaf_parent_pos = leaf_parent_pos;
+    T.h.parent_gid = r.parent_gid;
+    T.h.parent_leaf_pos = r.leaf_pos;
+    T.h.parent_leaf_idx = r.leaf_idx;
+    T.h.parent_leaf_parent_pos = r.leaf_parent_pos;
     T.nodes.push_back(svo_node{0, 0});
-    T.attribs.push_back(lv[root_level].attr[2 * (size_t)root_index]);
-    T.attribs.push_back(lv[root_level].attr[2 * (size_t)root_index + 1]);
-    std::deque<QueueItem> q;
+    T.attribs.push_back(lv[r.level].attr[2 * (size_t)r.index]);
+    T.attribs.push_back(lv[r.level].attr[2 * (size_t)r.index + 1]);
+    std::vector<QueueItem> q;  // FIFO: q[head..] are waiting
+    q.reserve(std::min<uint64_t>(N, tree_nodes) + 8);
+    size_t head = 0;
     std::vector<QueueItem> frontier;  // treelet leaves in local-index order
-    q.push_back(QueueItem{root_level, root_index, 0, 0, 0});
+    q.push_back(QueueItem{r.level, r.index, 0, 0, 0});
     uint32_t cur_idx = 1;
-    while (!q.empty()) {
-      const QueueItem e = q.front();
+    while (head < q.size()) {
+      const QueueItem e = q[head];
       bool expand;
       if (bfs) {
         // reference rule: stop while fewer than 9 slots remain
@@ -336,7 +391,7 @@ svo_status build_scene(uint32_t depth, std::vector<uint64_t>& codes,
       } else {
         expand = e.level < depth && (e.local == 0 || !cut[e.level][e.index]);
       }
-      q.pop_front();
+      ++head;
       if (!expand) { frontier.push_back(e); continue; }
       const uint32_t m = lv[e.level].mask[e.index];
       uint32_t child = lv[e.level].first[e.index];
@@ -357,32 +412,33 @@ svo_status build_scene(uint32_t depth, std::vector<uint64_t>& codes,
         ++cur_idx;
       }
     }
-    for (const QueueItem& e : q) frontier.push_back(e);
+    for (size_t i = head; i < q.size(); ++i) frontier.push_back(q[i]);
     T.h.num_nodes = cur_idx;
     T.h.num_leaves = (uint32_t)frontier.size();
-    std::vector<Incomplete> inc;
+    inc.clear();
     for (const QueueItem& e : frontier) {
       T.nodes[e.parent_local].masks |= 1u << (kLeafShift + e.child_idx);
       if (e.level < depth) inc.push_back(Incomplete{e.level, e.index, e.local, e.child_idx, e.parent_local});
     }
-    sc->treelets.push_back(std::move(T));
-    incomplete.push_back(std::move(inc));
   };
 
-  build_treelet(0, 0, 0, 0, 0, 0, 0);
-  std::deque<uint32_t> tq;
-  if (!incomplete[0].empty()) tq.push_back(0);
-  while (!tq.empty()) {
-    const uint32_t t = tq.front();
-    tq.pop_front();
-    std::vector<Incomplete> inc;
-    inc.swap(incomplete[t]);
-    for (const Incomplete& e : inc) {
-      const uint32_t gid = (uint32_t)sc->treelets.size();
-      build_treelet(gid, e.level, e.index, t, e.local, e.child_idx, e.parent_local);
-      sc->treelets[t].nodes[e.local].first_child = gid;
-      if (!incomplete[gid].empty()) tq.push_back(gid);
+  std::vector<Root> gen{Root{0, 0, 0, 0, 0, 0}};
+  while (!gen.empty()) {
+    const size_t base = sc->treelets.size();
+    sc->treelets.resize(base + gen.size());
+    std::vector<std::vector<Incomplete>> inc(gen.size());
+#pragma omp parallel for schedule(dynamic, 4) if (gen.size() >= 8)
+    for (int64_t g = 0; g < (int64_t)gen.size(); ++g) build_treelet((uint32_t)(base + g), gen[g], inc[g]);
+    std::vector<Root> next;
+    for (size_t g = 0; g < gen.size(); ++g) {
+      const uint32_t t = (uint32_t)(base + g);
+      for (const Incomplete& e : inc[g]) {
+        const uint32_t gid = (uint32_t)(base + gen.size() + next.size());
+        sc->treelets[t].nodes[e.local].first_child = gid;
+        next.push_back(Root{e.level, e.index, t, e.local, e.child_idx, e.parent_local});
+      }
     }
+    gen.swap(next);
   }
   *out = sc.release();
   return SVO_OK;
