@@ -249,11 +249,13 @@ def ooc_section(policy, frames, depth=13, treelet_bytes=131072, pool_bytes=8 << 
 
 
 def _ooc_pipelined(mm, cams, divide, policy, seq_pages, seq_resident_end):
-    """The same fly-through with frame i's render overlapping the analyse
-    pass and host feedback update of frame i+1 (second stream).  Both read
-    the residency frame i renders with, so every frame sees the same
-    residency as in the sequential loop; the upload of frame i+1 waits for
-    render i."""
+    """The same fly-through with frame i's render overlapping the host
+    residency update of frame i+1.  After the upload of frame i, the analyse
+    pass of frame i+1 runs and its records are read back before render i is
+    enqueued (a readback queued behind a running render waits for it); the
+    host then applies them while render i runs.  Both frames read the
+    residency frame i renders with, so every frame sees the same residency as
+    in the sequential loop; the upload of frame i+1 waits for render i."""
     import numpy as np
     import libsvo_amd as svo
     from libsvo_amd import workloads
@@ -271,10 +273,9 @@ def _ooc_pipelined(mm, cams, divide, policy, seq_pages, seq_resident_end):
         s_render.synchronize()  # render i-1 has finished with the pools
         mm.updateDeviceMemory(s_render)
         pages.append(st["inserted"])
-        # the small analyse launch goes first, so that it does not queue
-        # behind the render's waves
         if i + 1 < len(cams):
             mm.performAnalysePass(cams[i + 1], divide, jit[i + 1], policy, s_analyse)
+            mm.fetchFeedback(s_analyse)
         view[0].camera = cam
         r.render(view, policy, s_render)
         if i + 1 < len(cams):

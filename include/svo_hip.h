@@ -491,6 +491,15 @@ svo_status svo_mm_read_feedback(svo_mm* mm, void* stream,
                                 uint32_t* new_gids, float* new_errors,
                                 uint32_t new_cap, uint32_t* n_new);
 
+/* Reads the tables of the records gathered since the previous update back
+ * now, on `stream` (waiting for the launches on it), and ends the recording
+ * cycle; the next svo_mm_apply_feedback (or svo_mm_read_feedback) uses them
+ * instead of reading.  Unlike svo_mm_read_feedback, the visible sets are
+ * built later, against the slot table of that moment.  Lets a caller take
+ * the readback before it enqueues a render, whose waves a readback queued
+ * after them would wait for, and build the sets while the render runs. */
+svo_status svo_mm_fetch_feedback(svo_mm* mm, void* stream);
+
 /* The same update from caller-supplied aggregated records (no device
  * access; works on host-only handles): slots[i] was hit with largest error
  * slot_errors[i]; gids[i] (distinct, non-root) asked for with largest error

@@ -147,6 +147,7 @@ SIGNATURES = {
     "svo_mm_analyse": (ctypes.c_int, [vp, P(svo_camera), u32, f32, f32, ctypes.c_int, vp]),
     "svo_mm_apply_feedback": (ctypes.c_int, [vp, vp, P(svo_feedback_stats)]),
     "svo_mm_apply_records": (ctypes.c_int, [vp, vp, vp, u32, vp, vp, u32, P(svo_feedback_stats)]),
+    "svo_mm_fetch_feedback": (ctypes.c_int, [vp, vp]),
     "svo_mm_read_feedback": (ctypes.c_int, [vp, vp, vp, vp, u32, P(u32), vp, vp, u32, P(u32)]),
     "svo_server_create": (ctypes.c_int, [vp, P(svo_server_config), P(vp)]),
     "svo_server_destroy": (None, [vp]),

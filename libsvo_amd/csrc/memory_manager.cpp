@@ -138,6 +138,7 @@ struct svo_mm {
   // the records read back by the last table read, and whether the sets made
   // from them (fb_sets) wait for the next apply
   bool fb_cached = false;
+  bool fb_fetched = false;  // fb_seen / fb_asked hold tables read by svo_mm_fetch_feedback
   std::vector<std::pair<uint32_t, float>> fb_seen, fb_asked;
 
   bool host_only() const { return device < 0; }
@@ -674,6 +675,7 @@ extern "C" svo_status svo_mm_reset(svo_mm* mm) {
   if (!mm) { set_error("svo_mm_reset: null handle"); return SVO_ERR_ARG; }
   mm->reset_host();
   mm->fb_cached = false;
+  mm->fb_fetched = false;
   for (uint32_t s = 0; s < mm->num_slots; ++s) mm->mark(s);
   return mm->insert(0, 999.0f);
 }
@@ -985,8 +987,11 @@ extern "C" svo_status svo_mm_read_feedback(svo_mm* mm, void* stream, uint32_t* o
   }
   if (mm->host_only()) { set_error("svo_mm_read_feedback: handle has no device pools"); return SVO_ERR_STATE; }
   if (!mm->fb_cached) {
-    svo_status r = read_feedback_tables(mm, (hipStream_t)stream);
-    if (r != SVO_OK) return r;
+    if (!mm->fb_fetched) {
+      svo_status r = read_feedback_tables(mm, (hipStream_t)stream);
+      if (r != SVO_OK) return r;
+    }
+    mm->fb_fetched = false;
     mm->fb_sets = mm->make_sets(mm->fb_seen, mm->fb_asked);
     mm->fb_cached = true;
   }
@@ -1007,12 +1012,25 @@ extern "C" svo_status svo_mm_apply_feedback(svo_mm* mm, void* stream, svo_feedba
   if (!mm) { set_error("svo_mm_apply_feedback: null handle"); return SVO_ERR_ARG; }
   if (mm->host_only()) { set_error("svo_mm_apply_feedback: handle has no device pools"); return SVO_ERR_STATE; }
   if (!mm->fb_cached) {
-    svo_status r = read_feedback_tables(mm, (hipStream_t)stream);
-    if (r != SVO_OK) return r;
+    if (!mm->fb_fetched) {
+      svo_status r = read_feedback_tables(mm, (hipStream_t)stream);
+      if (r != SVO_OK) return r;
+    }
     mm->fb_sets = mm->make_sets(mm->fb_seen, mm->fb_asked);
   }
   mm->fb_cached = false;
+  mm->fb_fetched = false;
   mm->apply_sets(mm->fb_sets, stats);
+  return SVO_OK;
+}
+
+extern "C" svo_status svo_mm_fetch_feedback(svo_mm* mm, void* stream) {
+  if (!mm) { set_error("svo_mm_fetch_feedback: null handle"); return SVO_ERR_ARG; }
+  if (mm->host_only()) { set_error("svo_mm_fetch_feedback: handle has no device pools"); return SVO_ERR_STATE; }
+  if (mm->fb_cached || mm->fb_fetched) return SVO_OK;  // this cycle was read already
+  svo_status r = read_feedback_tables(mm, (hipStream_t)stream);
+  if (r != SVO_OK) return r;
+  mm->fb_fetched = true;
   return SVO_OK;
 }
 

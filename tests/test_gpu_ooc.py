@@ -71,6 +71,8 @@ def test_analyse_apply_render_matches_reference_bookkeeping(terrain8, policy):
         inserted = model.update_client_side(old, new)
         _say(f"frame {f} policy={policy}: oracle asks {_asked(rec)}, inserts {inserted}")
         mm.performAnalysePass(cam, 4, jit, policy, s)
+        if f % 2:  # a readback taken ahead (fetchFeedback) is the one later calls use
+            mm.fetchFeedback(s)
         vs = mm.readFeedback(s)  # RenderPassAnalyse's visible sets, then applied below
         assert vs["old"] == [(g, float(np.float32(e))) for g, e in old], f"visible old set, frame {f}"
         assert vs["new"] == [(g, float(np.float32(e))) for g, e in new], f"visible new set, frame {f}"
