@@ -381,7 +381,7 @@ typedef struct svo_launch_config {
 
 /* A renderer owns the device-side view descriptors of its launches (a ring,
  * so that launches on different streams can be in flight together) and the
- * launch shape.  Default shape: 16-wide tiles, 4 waves, LDS stack. */
+ * launch shape.  Default shape: 8-wide tiles, 1 wave, LDS stack. */
 typedef struct svo_renderer svo_renderer;
 svo_status svo_renderer_create(int device, svo_renderer** out);
 void svo_renderer_destroy(svo_renderer* r);

@@ -98,7 +98,7 @@ svo_status shape_from_config(const svo_launch_config* c, LaunchShape* s) {
 
 struct svo_renderer {
   int device = 0;
-  LaunchShape shape{16, 4, SVO_STACK_LDS};
+  LaunchShape shape{8, 1, SVO_STACK_LDS};
   Slot ring[kRing];
   uint32_t next = 0;
 
@@ -200,7 +200,7 @@ extern "C" void svo_renderer_destroy(svo_renderer* r) { delete r; }
 
 extern "C" svo_status svo_renderer_set_config(svo_renderer* r, const svo_launch_config* c) {
   if (!r) { set_error("svo_renderer_set_config: null renderer"); return SVO_ERR_ARG; }
-  if (!c) { r->shape = LaunchShape{16, 4, SVO_STACK_LDS}; return SVO_OK; }
+  if (!c) { r->shape = LaunchShape{8, 1, SVO_STACK_LDS}; return SVO_OK; }
   return shape_from_config(c, &r->shape);
 }
 

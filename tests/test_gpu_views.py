@@ -103,7 +103,7 @@ def test_every_launch_shape_bit_identical(scenes, policy):
     views = _views(scenes, seed=50)
     refs = [_oracle(v.mm, v.camera, policy) for v in views]
     r = svo.Renderer(0)
-    assert r.config == (16, 4, "lds")
+    assert r.config == (8, 1, "lds")
     for shape in svo.LAUNCH_SHAPES:
         r.config = shape
         assert r.config == shape
@@ -122,10 +122,10 @@ def test_bad_launch_config_rejected():
     for bad in [(12, 4, "lds"), (16, 3, "lds"), (16, 4, 7)]:
         with pytest.raises(svo.SvoError):
             r.config = bad
-    assert r.config == (16, 4, "lds")
+    assert r.config == (8, 1, "lds")
     r.config = (32, 1, "private")
     r.config = None
-    assert r.config == (16, 4, "lds")
+    assert r.config == (8, 1, "lds")
 
 
 def test_autotune_installs_a_shape_with_identical_results(scenes):
