@@ -91,3 +91,7 @@ def test_host_only_handle_refuses_render():
     arr = (_capi.svo_camera * 1)(cam)
     assert _capi.lib.svo_render(mm.handle, arr, 1, out, 0, None) == _capi.SVO_ERR_STATE
     assert "no device pools" in _capi.lib.svo_last_error().decode()
+    # the feedback readbacks need the device tables as well
+    assert _capi.lib.svo_mm_fetch_feedback(mm.handle, None) == _capi.SVO_ERR_STATE
+    assert "no device pools" in _capi.lib.svo_last_error().decode()
+    assert _capi.lib.svo_mm_fetch_feedback(None, None) == _capi.SVO_ERR_ARG
