@@ -123,7 +123,7 @@ struct svo_mm {
   uint32_t* d_fb_queue = nullptr;
   uint32_t* d_fb_count = nullptr;
   unsigned long long* h_fb_slot = nullptr;
-  unsigned long long* h_fb_gid = nullptr;  // the whole per-gid table, read back
+  unsigned long long* h_fb_gid = nullptr;  // key of each asked gid, by queue position
   uint32_t* h_zero = nullptr;               // a pinned 0 that resets the queue count
   uint32_t* h_fb_queue = nullptr;
   uint32_t* h_fb_count = nullptr;
@@ -942,22 +942,22 @@ extern "C" svo_status svo_mm_apply_records(svo_mm* mm, const uint32_t* slots, co
 namespace {
 
 // Reads this cycle's slot keys and asked gids back into mm->fb_seen /
-// fb_asked, then starts a new recording cycle.  Only copies (no kernel):
-// the slot keys, the queue of asked gids with its count and the per-gid
-// table come back whole, the queue count is reset from a pinned zero, and
-// the host picks each asked gid's key.  A copy does not wait for compute
-// units, so this overlaps with a render running on another stream.
+// fb_asked, then starts a new recording cycle.  One export kernel writes the
+// slot keys, the count and the asked gids with their keys into the pinned
+// buffers (only the queued entries of the per-gid table cross PCIe), then
+// the count is reset from a pinned zero.  The recording launches are waited
+// for first: work queued behind a running kernel was measured to wait for a
+// render on another stream as well (fly-through with overlapped renders:
+// 614 -> 800 frames/s), so callers that overlap renders take the readback
+// before they enqueue the render (svo_mm_fetch_feedback).
 svo_status read_feedback_tables(svo_mm* mm, hipStream_t st) {
   HIP_TRY(hipSetDevice(mm->device));
   const uint32_t ng = (uint32_t)mm->scene->treelets.size();
-  // Let the recording launches finish first: copies queued behind a running
-  // kernel were measured to wait for a render on another stream as well
-  // (fly-through with overlapped renders: 614 -> 800 frames/s).
   HIP_TRY(hipStreamSynchronize(st));
-  HIP_TRY(hipMemcpyAsync(mm->h_fb_slot, mm->d_fb_slot, (size_t)mm->num_slots * 8, hipMemcpyDeviceToHost, st));
-  HIP_TRY(hipMemcpyAsync(mm->h_fb_count, mm->d_fb_count, 4, hipMemcpyDeviceToHost, st));
-  HIP_TRY(hipMemcpyAsync(mm->h_fb_queue, mm->d_fb_queue, (size_t)ng * 4, hipMemcpyDeviceToHost, st));
-  HIP_TRY(hipMemcpyAsync(mm->h_fb_gid, mm->d_fb_gid, (size_t)ng * 8, hipMemcpyDeviceToHost, st));
+  const FeedbackExport x{mm->d_fb_slot, mm->d_fb_count, mm->d_fb_queue, mm->d_fb_gid, mm->num_slots, ng,
+                         mm->h_fb_slot, mm->h_fb_count, mm->h_fb_queue, mm->h_fb_gid};
+  const int e = launch_feedback_export(x, st);
+  if (e != 0) { set_error("feedback readback launch failed: %s", hipGetErrorString((hipError_t)e)); return SVO_ERR_HIP; }
   HIP_TRY(hipMemcpyAsync(mm->d_fb_count, mm->h_zero, 4, hipMemcpyHostToDevice, st));
   HIP_TRY(hipStreamSynchronize(st));
   const uint32_t n = std::min(*mm->h_fb_count, ng);
@@ -970,7 +970,7 @@ svo_status read_feedback_tables(svo_mm* mm, hipStream_t st) {
   }
   for (uint32_t i = 0; i < n; ++i) {
     const uint32_t g = mm->h_fb_queue[i];
-    if (g < ng) mm->fb_asked.push_back({g, key_error(mm->h_fb_gid[g])});
+    if (g < ng) mm->fb_asked.push_back({g, key_error(mm->h_fb_gid[i])});
   }
   ++mm->fb_stamp;  // records of this cycle are now stale
   return SVO_OK;

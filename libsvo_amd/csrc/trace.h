@@ -105,4 +105,22 @@ struct UploadSpan {
 // fixed cost per command.
 int launch_upload(const UploadSpan* spans, uint32_t num_spans, uint64_t total_chunks, void* stream);
 
+// Device feedback tables of one residency handle and the pinned host buffers
+// a readback fills (h_keys[i] = key of the gid in h_queue[i]).
+struct FeedbackExport {
+  const unsigned long long* fb_slot;
+  const uint32_t* fb_count;
+  const uint32_t* fb_queue;
+  const unsigned long long* fb_gid;
+  uint32_t num_slots;
+  uint32_t num_gids;
+  unsigned long long* h_slot;
+  uint32_t* h_count;
+  uint32_t* h_queue;
+  unsigned long long* h_keys;
+};
+
+// Enqueues the readback of one analysis cycle into the pinned buffers.
+int launch_feedback_export(const FeedbackExport& x, void* stream);
+
 }  // namespace svo_impl

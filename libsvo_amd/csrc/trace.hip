@@ -900,6 +900,25 @@ __global__ __launch_bounds__(256) void upload_kernel(const UploadSpan* __restric
   }
 }
 
+// Readback of one analysis cycle into pinned host memory: the slot keys
+// whole, the count of asked gids as recorded, and the first min(count,
+// num_gids) queue entries with the key of each (the per-gid table stays on
+// the device).  Stores go straight over PCIe; the host reads the buffers
+// after a stream sync.
+__global__ __launch_bounds__(256) void feedback_export_kernel(FeedbackExport x) {
+  const uint32_t stride = gridDim.x * blockDim.x;
+  const uint32_t t0 = blockIdx.x * blockDim.x + threadIdx.x;
+  for (uint32_t s = t0; s < x.num_slots; s += stride) x.h_slot[s] = x.fb_slot[s];
+  const uint32_t count = *x.fb_count;
+  const uint32_t n = count < x.num_gids ? count : x.num_gids;
+  for (uint32_t i = t0; i < n; i += stride) {
+    const uint32_t g = x.fb_queue[i];
+    x.h_queue[i] = g;
+    x.h_keys[i] = g < x.num_gids ? x.fb_gid[g] : 0ull;
+  }
+  if (t0 == 0) *x.h_count = count;
+}
+
 using TraceFn = void (*)(const TraceFrame*, uint32_t, uint32_t);
 
 template <int POLICY, int TW, int WAVES>
@@ -947,6 +966,13 @@ int launch_upload(const UploadSpan* spans, uint32_t num_spans, uint64_t total_ch
   const unsigned blocks = (unsigned)std::min<uint64_t>((total_chunks + 255) / 256, 2048);
   (void)hipGetLastError();  // see launch_trace
   hipLaunchKernelGGL(upload_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream, spans, num_spans, total_chunks);
+  return (int)hipGetLastError();
+}
+
+int launch_feedback_export(const FeedbackExport& x, void* stream) {
+  const unsigned blocks = std::max(1u, std::min((x.num_slots + 255u) / 256u, 64u));
+  (void)hipGetLastError();  // see launch_trace
+  hipLaunchKernelGGL(feedback_export_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream, x);
   return (int)hipGetLastError();
 }
 
