@@ -388,7 +388,7 @@ def main():
     frame = svo.Frame.alloc(cam.width, cam.height)
     view = [svo.View(mm, cam, frame)]
     renderer = svo.Renderer(0)
-    shape, tune_ms = renderer.autotune(view, args.policy, stream, iters=5)
+    shape, tune_ms = renderer.autotune(view, args.policy, stream, iters=20)  # shapes lie within a few %
     log(f"autotuned launch shape {shape} ({tune_ms:.4f} ms in tuning)")
     if args.warmup > 0:
         renderer.timed(view, args.policy, stream, iters=args.warmup)
@@ -404,7 +404,7 @@ def main():
 
     # the other traversal policy on the same frame (autotuned on its own)
     other = "cpu3" if args.policy == "treelet" else "treelet"
-    oshape, _ = renderer.autotune(view, other, stream, iters=3)
+    oshape, _ = renderer.autotune(view, other, stream, iters=20)
     ms_other = renderer.timed(view, other, stream, iters=max(10, args.steps // 2))
     log(f"{other} policy: {oshape} {ms_other:.4f} ms/launch = {rays / ms_other / 1e3:.1f} Mrays/s")
     policies = {args.policy: {"mrays_per_s": round(value, 1), "ms_per_launch": round(ms, 5), "launch_shape": list(shape)},
