@@ -250,12 +250,13 @@ def ooc_section(policy, frames, depth=13, treelet_bytes=131072, pool_bytes=8 << 
 
 def _ooc_pipelined(mm, cams, divide, policy, seq_pages, seq_resident_end):
     """The same fly-through with frame i's render overlapping the host
-    residency update of frame i+1.  After the upload of frame i, the analyse
-    pass of frame i+1 runs and its records are read back before render i is
-    enqueued (a readback queued behind a running render waits for it); the
-    host then applies them while render i runs.  Both frames read the
-    residency frame i renders with, so every frame sees the same residency as
-    in the sequential loop; the upload of frame i+1 waits for render i."""
+    residency update of frame i+1, on one stream: after the upload of frame
+    i (the host waits for it), the analyse pass of frame i+1, the readback of
+    its records (fetchFeedback) and render i are enqueued back to back; the
+    host waits for the readback only and applies the records while render i
+    runs.  Both frames read the residency frame i renders with, so every
+    frame sees the same residency as in the sequential loop; the upload of
+    frame i+1 follows render i on the stream."""
     import numpy as np
     import libsvo_amd as svo
     from libsvo_amd import workloads
@@ -263,24 +264,23 @@ def _ooc_pipelined(mm, cams, divide, policy, seq_pages, seq_resident_end):
     jit = [tuple(float(v) for v in rng.uniform(-1.0, 1.0, 2)) for _ in cams]
     frame = svo.Frame.alloc(cams[0].width, cams[0].height)
     r = svo.Renderer(0)
-    s_render, s_analyse = svo.Stream(), svo.Stream()
+    s = svo.Stream()
     view = [svo.View(mm, cams[0], frame)]
     pages = []
     wall0 = time.perf_counter()
-    mm.performAnalysePass(cams[0], divide, jit[0], policy, s_analyse)
-    st = mm.updateClientSideIncoreBuffer(s_analyse)
+    mm.performAnalysePass(cams[0], divide, jit[0], policy, s)
+    st = mm.updateClientSideIncoreBuffer(s)
     for i, cam in enumerate(cams):
-        s_render.synchronize()  # render i-1 has finished with the pools
-        mm.updateDeviceMemory(s_render)
+        mm.updateDeviceMemory(s)  # after render i-1 on the stream
         pages.append(st["inserted"])
         if i + 1 < len(cams):
-            mm.performAnalysePass(cams[i + 1], divide, jit[i + 1], policy, s_analyse)
-            mm.fetchFeedback(s_analyse)
+            mm.performAnalysePass(cams[i + 1], divide, jit[i + 1], policy, s)
+            mm.fetchFeedback(s)
         view[0].camera = cam
-        r.render(view, policy, s_render)
+        r.render(view, policy, s)
         if i + 1 < len(cams):
-            st = mm.updateClientSideIncoreBuffer(s_analyse)
-    s_render.synchronize()
+            st = mm.updateClientSideIncoreBuffer(s)
+    s.synchronize()
     wall = time.perf_counter() - wall0
     same = pages == list(seq_pages) and mm.info["resident_treelets"] == seq_resident_end
     return {"frames_per_s": round(len(cams) / wall, 1), "ms_per_frame": round(wall * 1e3 / len(cams), 3),

@@ -491,13 +491,13 @@ svo_status svo_mm_read_feedback(svo_mm* mm, void* stream,
                                 uint32_t* new_gids, float* new_errors,
                                 uint32_t new_cap, uint32_t* n_new);
 
-/* Reads the tables of the records gathered since the previous update back
- * now, on `stream` (waiting for the launches on it), and ends the recording
- * cycle; the next svo_mm_apply_feedback (or svo_mm_read_feedback) uses them
- * instead of reading.  Unlike svo_mm_read_feedback, the visible sets are
- * built later, against the slot table of that moment.  Lets a caller take
- * the readback before it enqueues a render, whose waves a readback queued
- * after them would wait for, and build the sets while the render runs. */
+/* Enqueues the readback of the records gathered since the previous update
+ * on `stream`, behind the launches that recorded them, and ends the
+ * recording cycle (later recordings start a new one); returns at once.  The
+ * next svo_mm_apply_feedback (or svo_mm_read_feedback) waits for it instead
+ * of reading, and builds the visible sets then, against the slot table of
+ * that moment.  Lets a caller enqueue the readback and its render on one
+ * stream and do the host update while the render runs. */
 svo_status svo_mm_fetch_feedback(svo_mm* mm, void* stream);
 
 /* The same update from caller-supplied aggregated records (no device

@@ -285,11 +285,12 @@ class TreeletMemoryManagerHip:
         return {k: getattr(st, k) for k, _ in _capi.svo_feedback_stats._fields_}
 
     def fetchFeedback(self, stream: Optional["Stream"] = None) -> None:
-        """Reads the records gathered since the last update back from the
-        device now (waiting for the launches on `stream` that recorded them);
-        the next updateClientSideIncoreBuffer applies them without reading
-        again and builds the visible sets then.  Lets a caller take the
-        readback before it enqueues a render."""
+        """Enqueues the readback of the records gathered since the last
+        update on `stream` (behind the launches that recorded them) and
+        returns at once; the next updateClientSideIncoreBuffer waits for it
+        instead of reading and builds the visible sets then.  Lets a caller
+        enqueue its render behind the readback and update the residency
+        while the render runs."""
         check(lib.svo_mm_fetch_feedback(self._h, stream.handle if stream else None), "svo_mm_fetch_feedback")
 
     def readFeedback(self, stream: Optional["Stream"] = None) -> Dict[str, List[Tuple[int, float]]]:

@@ -138,7 +138,13 @@ struct svo_mm {
   // the records read back by the last table read, and whether the sets made
   // from them (fb_sets) wait for the next apply
   bool fb_cached = false;
-  bool fb_fetched = false;  // fb_seen / fb_asked hold tables read by svo_mm_fetch_feedback
+  // Readback states of a cycle: an export enqueued by svo_mm_fetch_feedback
+  // (fb_pending, finished by fb_event), its tables parsed into fb_seen /
+  // fb_asked (fb_parsed), the visible sets built from them (fb_cached).
+  bool fb_pending = false;
+  bool fb_parsed = false;
+  uint32_t fb_export_stamp = 0;  // stamp of the cycle the export reads
+  hipEvent_t fb_event = nullptr;
   std::vector<std::pair<uint32_t, float>> fb_seen, fb_asked;
 
   bool host_only() const { return device < 0; }
@@ -150,6 +156,7 @@ struct svo_mm {
     hipSetDevice(device);
     if (renderer) svo_renderer_destroy(renderer);
     hipDeviceSynchronize();
+    if (fb_event) hipEventDestroy(fb_event);
     hipFree(d_nodes);
     hipFree(d_colors);
     hipFree(d_normals);
@@ -648,6 +655,7 @@ extern "C" svo_status svo_mm_create(const svo_scene* scene, uint64_t incore_byte
   HIP_TRY(hipMemset(mm->d_fb_slot, 0, slots * 8));
   HIP_TRY(hipMemset(mm->d_fb_gid, 0, ng * 8));
   HIP_TRY(hipMemset(mm->d_fb_count, 0, 4));
+  HIP_TRY(hipEventCreateWithFlags(&mm->fb_event, hipEventDisableTiming));
   *mm->h_zero = 0u;
   mm->span_cap = (uint32_t)(3 * slots + 1);  // three spans per slot, plus the slot table
   if (hipHostMalloc(&mm->h_spans, (size_t)mm->span_cap * sizeof(UploadSpan), 0) != hipSuccess ||
@@ -674,8 +682,10 @@ extern "C" void svo_mm_destroy(svo_mm* mm) { delete mm; }
 extern "C" svo_status svo_mm_reset(svo_mm* mm) {
   if (!mm) { set_error("svo_mm_reset: null handle"); return SVO_ERR_ARG; }
   mm->reset_host();
+  if (mm->fb_pending) hipEventSynchronize(mm->fb_event);  // its pinned buffers stay in use until then
   mm->fb_cached = false;
-  mm->fb_fetched = false;
+  mm->fb_pending = false;
+  mm->fb_parsed = false;
   for (uint32_t s = 0; s < mm->num_slots; ++s) mm->mark(s);
   return mm->insert(0, 999.0f);
 }
@@ -941,27 +951,34 @@ extern "C" svo_status svo_mm_apply_records(svo_mm* mm, const uint32_t* slots, co
 
 namespace {
 
-// Reads this cycle's slot keys and asked gids back into mm->fb_seen /
-// fb_asked, then starts a new recording cycle.  One export kernel writes the
-// slot keys, the count and the asked gids with their keys into the pinned
-// buffers (only the queued entries of the per-gid table cross PCIe), then
-// the count is reset from a pinned zero.  The recording launches are waited
-// for first: work queued behind a running kernel was measured to wait for a
-// render on another stream as well (fly-through with overlapped renders:
-// 614 -> 800 frames/s), so callers that overlap renders take the readback
-// before they enqueue the render (svo_mm_fetch_feedback).
-svo_status read_feedback_tables(svo_mm* mm, hipStream_t st) {
+// Readback of a recording cycle, in two halves.  enqueue_export: one
+// export kernel, behind the recording launches on `st`, writes the slot
+// keys, the count and the asked gids with their keys into the pinned
+// buffers (only the queued entries of the per-gid table cross PCIe), the
+// count is reset from a pinned zero, and the cycle ends: later recordings
+// carry the next stamp.  finish_export waits for that work and parses the
+// buffers into mm->fb_seen / fb_asked.
+svo_status enqueue_export(svo_mm* mm, hipStream_t st) {
   HIP_TRY(hipSetDevice(mm->device));
   const uint32_t ng = (uint32_t)mm->scene->treelets.size();
-  HIP_TRY(hipStreamSynchronize(st));
   const FeedbackExport x{mm->d_fb_slot, mm->d_fb_count, mm->d_fb_queue, mm->d_fb_gid, mm->num_slots, ng,
                          mm->h_fb_slot, mm->h_fb_count, mm->h_fb_queue, mm->h_fb_gid};
   const int e = launch_feedback_export(x, st);
   if (e != 0) { set_error("feedback readback launch failed: %s", hipGetErrorString((hipError_t)e)); return SVO_ERR_HIP; }
   HIP_TRY(hipMemcpyAsync(mm->d_fb_count, mm->h_zero, 4, hipMemcpyHostToDevice, st));
-  HIP_TRY(hipStreamSynchronize(st));
+  HIP_TRY(hipEventRecord(mm->fb_event, st));
+  mm->fb_export_stamp = mm->fb_stamp;
+  ++mm->fb_stamp;  // records of this cycle are now stale
+  mm->fb_pending = true;
+  return SVO_OK;
+}
+
+svo_status finish_export(svo_mm* mm) {
+  HIP_TRY(hipEventSynchronize(mm->fb_event));
+  mm->fb_pending = false;
+  const uint32_t ng = (uint32_t)mm->scene->treelets.size();
   const uint32_t n = std::min(*mm->h_fb_count, ng);
-  const uint32_t stamp = mm->fb_stamp;
+  const uint32_t stamp = mm->fb_export_stamp;
   mm->fb_seen.clear();
   mm->fb_asked.clear();
   for (uint32_t s = 0; s < mm->num_slots; ++s) {
@@ -972,8 +989,26 @@ svo_status read_feedback_tables(svo_mm* mm, hipStream_t st) {
     const uint32_t g = mm->h_fb_queue[i];
     if (g < ng) mm->fb_asked.push_back({g, key_error(mm->h_fb_gid[i])});
   }
-  ++mm->fb_stamp;  // records of this cycle are now stale
+  mm->fb_parsed = true;
   return SVO_OK;
+}
+
+// The cycle's tables in mm->fb_seen / fb_asked: parsed already, or the
+// pending export finished, or a readback now.  A readback of its own waits
+// for the recording launches before it enqueues the export: work queued
+// behind a running kernel was measured to wait for a render on another
+// stream as well (fly-through with overlapped renders: 614 -> 800
+// frames/s); callers that overlap renders use svo_mm_fetch_feedback before
+// they enqueue the render.
+svo_status parsed_tables(svo_mm* mm, hipStream_t st) {
+  if (mm->fb_parsed) return SVO_OK;
+  if (!mm->fb_pending) {
+    HIP_TRY(hipSetDevice(mm->device));
+    HIP_TRY(hipStreamSynchronize(st));
+    svo_status r = enqueue_export(mm, st);
+    if (r != SVO_OK) return r;
+  }
+  return finish_export(mm);
 }
 
 }  // namespace
@@ -987,11 +1022,9 @@ extern "C" svo_status svo_mm_read_feedback(svo_mm* mm, void* stream, uint32_t* o
   }
   if (mm->host_only()) { set_error("svo_mm_read_feedback: handle has no device pools"); return SVO_ERR_STATE; }
   if (!mm->fb_cached) {
-    if (!mm->fb_fetched) {
-      svo_status r = read_feedback_tables(mm, (hipStream_t)stream);
-      if (r != SVO_OK) return r;
-    }
-    mm->fb_fetched = false;
+    svo_status r = parsed_tables(mm, (hipStream_t)stream);
+    if (r != SVO_OK) return r;
+    mm->fb_parsed = false;
     mm->fb_sets = mm->make_sets(mm->fb_seen, mm->fb_asked);
     mm->fb_cached = true;
   }
@@ -1012,14 +1045,12 @@ extern "C" svo_status svo_mm_apply_feedback(svo_mm* mm, void* stream, svo_feedba
   if (!mm) { set_error("svo_mm_apply_feedback: null handle"); return SVO_ERR_ARG; }
   if (mm->host_only()) { set_error("svo_mm_apply_feedback: handle has no device pools"); return SVO_ERR_STATE; }
   if (!mm->fb_cached) {
-    if (!mm->fb_fetched) {
-      svo_status r = read_feedback_tables(mm, (hipStream_t)stream);
-      if (r != SVO_OK) return r;
-    }
+    svo_status r = parsed_tables(mm, (hipStream_t)stream);
+    if (r != SVO_OK) return r;
     mm->fb_sets = mm->make_sets(mm->fb_seen, mm->fb_asked);
   }
   mm->fb_cached = false;
-  mm->fb_fetched = false;
+  mm->fb_parsed = false;
   mm->apply_sets(mm->fb_sets, stats);
   return SVO_OK;
 }
@@ -1027,11 +1058,8 @@ extern "C" svo_status svo_mm_apply_feedback(svo_mm* mm, void* stream, svo_feedba
 extern "C" svo_status svo_mm_fetch_feedback(svo_mm* mm, void* stream) {
   if (!mm) { set_error("svo_mm_fetch_feedback: null handle"); return SVO_ERR_ARG; }
   if (mm->host_only()) { set_error("svo_mm_fetch_feedback: handle has no device pools"); return SVO_ERR_STATE; }
-  if (mm->fb_cached || mm->fb_fetched) return SVO_OK;  // this cycle was read already
-  svo_status r = read_feedback_tables(mm, (hipStream_t)stream);
-  if (r != SVO_OK) return r;
-  mm->fb_fetched = true;
-  return SVO_OK;
+  if (mm->fb_cached || mm->fb_parsed || mm->fb_pending) return SVO_OK;  // this cycle is read already
+  return enqueue_export(mm, (hipStream_t)stream);
 }
 
 // ------------------------------------------------------------ device utilities
