@@ -133,3 +133,35 @@ def test_feedback_without_records_changes_nothing_but_visibility(terrain8):
     mm.performAnalysePass(cam, 4)
     st = mm.updateClientSideIncoreBuffer()
     assert st["visible_slots"] == 0 and st["requests"] == 0
+
+
+def test_fetched_readback_states(terrain8):
+    """svo_mm_fetch_feedback enqueues the readback only: a second fetch of the
+    cycle does nothing, an update waiting on another stream gets the same
+    records as a plain update, and a reset drops a pending readback."""
+    sc, _ = terrain8
+    tb = sc.info["treelet_bytes"]
+    cams = _path(3, 320, 200, lod=0.25)
+    s1, s2 = svo.Stream(), svo.Stream()
+    ref = svo.TreeletMemoryManagerHip(sc, 48 * tb, device=0)
+    mm = svo.TreeletMemoryManagerHip(sc, 48 * tb, device=0)
+    for f, cam in enumerate(cams):
+        _say(f"frame {f}: plain update against fetch, fetch, update on another stream")
+        ref.performAnalysePass(cam, 4, (0.0, 0.0), "treelet", s1)
+        want = ref.updateClientSideIncoreBuffer(s1)
+        ref.updateDeviceMemory(s1)
+        mm.performAnalysePass(cam, 4, (0.0, 0.0), "treelet", s1)
+        mm.fetchFeedback(s1)
+        mm.fetchFeedback(s1)  # the cycle is read already: nothing more is enqueued
+        got = mm.updateClientSideIncoreBuffer(s2)
+        mm.updateDeviceMemory(s1)
+        assert got == want, f"frame {f}"
+        assert mm.getSlots() == ref.getSlots(), f"frame {f}"
+    # a reset with a readback pending: the next update reads a new cycle
+    mm.performAnalysePass(cams[0], 4, (0.0, 0.0), "treelet", s1)
+    mm.fetchFeedback(s1)
+    mm.resetIncoreBuffer()
+    mm.updateDeviceMemory(s1)
+    st = mm.updateClientSideIncoreBuffer(s1)
+    assert st["requests"] == 0 and st["inserted"] == 0
+    assert mm.info["resident_treelets"] == 1
