@@ -497,7 +497,9 @@ svo_status svo_mm_read_feedback(svo_mm* mm, void* stream,
  * next svo_mm_apply_feedback (or svo_mm_read_feedback) waits for it instead
  * of reading, and builds the visible sets then, against the slot table of
  * that moment.  Lets a caller enqueue the readback and its render on one
- * stream and do the host update while the render runs. */
+ * stream and do the host update while the render runs.  The reference reads
+ * its feedback buffer back with a blocking read inside performAnalysePass
+ * (src/RenderPassAnalyse.cpp:228-232); this splits that read off. */
 svo_status svo_mm_fetch_feedback(svo_mm* mm, void* stream);
 
 /* The same update from caller-supplied aggregated records (no device
