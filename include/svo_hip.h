@@ -256,7 +256,10 @@ typedef struct svo_upload_stats {
                                  reference's enqueueWrite calls move)   */
   uint64_t bytes_moved;       /* bytes actually sent: within each range
                                  only the part of a slot that is, or on
-                                 the device was, not zero padding       */
+                                 the device was, not zero padding; nodes
+                                 and colours (normals follow with the
+                                 first launch whose view shades with
+                                 them, or a device readback)            */
 } svo_upload_stats;
 
 /* TreeletMemoryManagerCl(path, incoreBufferSizeInByte, ctx) +

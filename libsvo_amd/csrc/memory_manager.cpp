@@ -106,6 +106,14 @@ struct svo_mm {
   // mark; everything after it is zero) in the host buffers and on the
   // device: fills and uploads skip the zero padding of partial treelets.
   std::vector<uint32_t> extent, dev_extent;
+  // Normals go up separately: only the view modes that shade with them
+  // read them on the device.  A refilled slot's normals are stale there
+  // (nrm_stale, listed in nrm_slots) until a launch with such a view, or a
+  // device readback of attributes, sends them (sync_normals);
+  // dev_nrm_extent is their device high-water mark.
+  std::vector<uint32_t> dev_nrm_extent;
+  std::vector<uint8_t> nrm_stale;
+  std::vector<uint32_t> nrm_slots;
   // What the next upload must send: slots whose content was replaced (a
   // fill or a reset) go up within their extents; otherwise only the nodes
   // a stitch or unstitch changed (global node indices).
@@ -535,11 +543,14 @@ struct svo_mm {
         if (!refilled[sl]) continue;
         const size_t n0 = (size_t)sl * N, nn = std::max(extent[sl], dev_extent[sl]);
         dev_extent[sl] = extent[sl];
+        if (!nrm_stale[sl]) {
+          nrm_stale[sl] = 1;
+          nrm_slots.push_back(sl);
+        }
         if (nn == 0) continue;
         add(incore.data() + n0, d_nodes + n0, nn * 8);
         add(colors.data() + n0, d_colors + n0, nn * 4);
-        add(normals.data() + n0, d_normals + n0, nn * 4);
-        moved += nn * 16;
+        moved += nn * 12;
       }
     }
     std::sort(patched.begin(), patched.end());
@@ -558,6 +569,32 @@ struct svo_mm {
     HIP_TRY(hipStreamSynchronize(st));
     for (const auto& r : runs) std::fill(refilled.begin() + r.first, refilled.begin() + r.first + r.second, 0u);
     patched.clear();
+    return SVO_OK;
+  }
+
+  // Sends the stale normals (see nrm_stale) in one upload-kernel launch on
+  // `st` and waits for it.
+  svo_status sync_normals(hipStream_t st) {
+    if (nrm_slots.empty() || host_only()) return SVO_OK;
+    if (nrm_slots.size() + 1 > span_cap) { set_error("normals upload: %zu spans", nrm_slots.size()); return SVO_ERR_STATE; }
+    uint32_t n = 0;
+    uint64_t chunks = 0;
+    for (const uint32_t sl : nrm_slots) {
+      nrm_stale[sl] = 0;
+      const size_t n0 = (size_t)sl * N, nn = std::max(extent[sl], dev_nrm_extent[sl]);
+      dev_nrm_extent[sl] = extent[sl];
+      if (nn == 0) continue;
+      h_spans[n++] = UploadSpan{(uint64_t)(uintptr_t)(normals.data() + n0), (uint64_t)(uintptr_t)(d_normals + n0), nn * 4,
+                                chunks};
+      chunks += (nn * 4 + 15) / 16;
+    }
+    nrm_slots.clear();
+    if (n == 0) return SVO_OK;
+    HIP_TRY(hipSetDevice(device));
+    HIP_TRY(hipMemcpyAsync(d_spans, h_spans, (size_t)n * sizeof(UploadSpan), hipMemcpyHostToDevice, st));
+    const int e = launch_upload(d_spans, n, chunks, st);
+    if (e != 0) { set_error("normals upload launch failed: %s", hipGetErrorString((hipError_t)e)); return SVO_ERR_HIP; }
+    HIP_TRY(hipStreamSynchronize(st));
     return SVO_OK;
   }
 
@@ -670,6 +707,8 @@ extern "C" svo_status svo_mm_create(const svo_scene* scene, uint64_t incore_byte
   HIP_TRY(hipMemset(mm->d_colors, 0, pool_nodes * 4));
   HIP_TRY(hipMemset(mm->d_normals, 0, pool_nodes * 4));
   mm->dev_extent.assign(mm->num_slots, 0u);
+  mm->dev_nrm_extent.assign(mm->num_slots, 0u);
+  mm->nrm_stale.assign(mm->num_slots, 0u);
   r = mm->upload_runs({{0u, mm->num_slots}}, nullptr, nullptr, nullptr);
   if (r != SVO_OK) return r;
   mm->dirty.clear();
@@ -770,6 +809,10 @@ extern "C" svo_status svo_mm_read_device_incore(const svo_mm* mm, uint64_t first
   HIP_TRY(hipSetDevice(mm->device));
   if (nodes) HIP_TRY(hipMemcpy(nodes, mm->d_nodes + first, count * 8, hipMemcpyDeviceToHost));
   if (attribs) {
+    // bring the device normals up to date first (the handle's observable
+    // state does not change)
+    svo_status r = const_cast<svo_mm*>(mm)->sync_normals(nullptr);
+    if (r != SVO_OK) return r;
     std::vector<uint32_t> n(count), c(count);
     HIP_TRY(hipMemcpy(n.data(), mm->d_normals + first, count * 4, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(c.data(), mm->d_colors + first, count * 4, hipMemcpyDeviceToHost));
@@ -783,6 +826,8 @@ extern "C" svo_status svo_mm_read_device_incore(const svo_mm* mm, uint64_t first
 namespace svo_impl {
 
 int mm_device(const svo_mm* mm) { return mm->device; }
+
+svo_status mm_sync_normals(svo_mm* mm, void* stream) { return mm->sync_normals((hipStream_t)stream); }
 
 svo_renderer* mm_default_renderer(svo_mm* mm) {
   if (!mm->renderer && svo_renderer_create(mm->device, &mm->renderer) != SVO_OK) return nullptr;

@@ -37,6 +37,7 @@ svo_status stage_views(const svo_view* views, uint32_t n, int policy, const Laun
   if (!views || n == 0 || n > kMaxViews) { set_error("need 1..%u views", kMaxViews); return SVO_ERR_ARG; }
   const uint32_t tw = shape.tile_w, th = 64u / shape.tile_w;
   out->frames.resize(n);
+  out->normal_handles.clear();
   uint64_t tiles = 0;
   uint32_t levels = 1;
   bool shade = false;
@@ -63,11 +64,22 @@ svo_status stage_views(const svo_view* views, uint32_t n, int policy, const Laun
     tiles += (uint64_t)F.tiles_x * ((c.height + th - 1) / th);
     levels = std::max(levels, F.pools.stack_levels);
     shade = shade || v.out.shade != nullptr;
+    if (v.out.shade && (c.view_mode == SVO_VIEW_NORMAL || c.view_mode == SVO_VIEW_PHONG) &&
+        std::find(out->normal_handles.begin(), out->normal_handles.end(), v.mm) == out->normal_handles.end())
+      out->normal_handles.push_back(v.mm);
   }
   if (tiles > 0x7FFFFFFFull) { set_error("launch of %llu tiles too large", (unsigned long long)tiles); return SVO_ERR_ARG; }
   out->num_tiles = (uint32_t)tiles;
   out->stack_levels = levels;
   out->shade = shade;
+  return SVO_OK;
+}
+
+svo_status sync_staged_normals(const StagedLaunch& sg, void* stream) {
+  for (svo_mm* mm : sg.normal_handles) {
+    const svo_status r = mm_sync_normals(mm, stream);
+    if (r != SVO_OK) return r;
+  }
   return SVO_OK;
 }
 
@@ -145,6 +157,8 @@ struct svo_renderer {
   }
 
   svo_status launch(const LaunchShape& sh, const StagedLaunch& sg, Slot* s, int policy, hipStream_t st) {
+    const svo_status ns = sync_staged_normals(sg, st);
+    if (ns != SVO_OK) return ns;
     int e = launch_trace(sh, s->d, (uint32_t)sg.frames.size(), sg.num_tiles, sg.stack_levels, policy, sg.shade, st);
     if (e != 0) { set_error("trace launch failed: %s", hipGetErrorString((hipError_t)e)); return SVO_ERR_HIP; }
     HIP_TRY(hipEventRecord(s->done, st));
@@ -337,6 +351,8 @@ extern "C" svo_status svo_batch_set_cameras(svo_batch* b, const svo_camera* cams
 extern "C" svo_status svo_batch_launch(svo_batch* b, void* stream) {
   if (!b) { set_error("svo_batch_launch: null batch"); return SVO_ERR_ARG; }
   HIP_TRY(hipSetDevice(b->device));
+  const svo_status ns = sync_staged_normals(b->sg, stream);
+  if (ns != SVO_OK) return ns;
   HIP_TRY(hipGraphLaunch(b->exec, (hipStream_t)stream));
   return SVO_OK;
 }

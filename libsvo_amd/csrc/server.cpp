@@ -255,6 +255,7 @@ struct svo_server {
       }
       std::copy(staged.frames.begin(), staged.frames.end(), f.h);
       HIP_TRY(hipMemcpyAsync(f.d, f.h, n * sizeof(TraceFrame), hipMemcpyHostToDevice, f.stream));
+      if ((st = sync_staged_normals(staged, f.stream)) != SVO_OK) return st;
       int e = launch_trace(shape, f.d, n, staged.num_tiles, staged.stack_levels, cfg.policy, staged.shade, f.stream);
       if (e != 0) { set_error("render launch failed: %s", hipGetErrorString((hipError_t)e)); return SVO_ERR_HIP; }
       HIP_TRY(hipEventRecord(f.done, f.stream));

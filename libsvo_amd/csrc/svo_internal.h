@@ -91,6 +91,9 @@ svo_renderer* mm_default_renderer(svo_mm* mm);
 // Pools and traversal parameters of `mm` for `policy`; fails while slots
 // wait for upload or when the handle has no device pools.
 svo_status mm_trace_pools(const svo_mm* mm, int policy, TracePools* out);
+// Sends the normals that uploads left stale (host-synchronous on `stream`);
+// launches whose views shade with normals call it first.
+svo_status mm_sync_normals(svo_mm* mm, void* stream);
 
 // Builds the scene from sorted, unique leaf codes at `depth` and two
 // attribute words per leaf.  Consumes `codes` (cleared on return).

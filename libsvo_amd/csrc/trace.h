@@ -75,7 +75,13 @@ struct StagedLaunch {
   uint32_t num_tiles = 0;
   uint32_t stack_levels = 1;
   bool shade = false;  // some view has a view-mode output
+  // residency handles of the views that shade with normals, each once
+  std::vector<svo_mm*> normal_handles;
 };
+
+// Sends the stale normals of a staged launch's handles (see
+// mm_sync_normals) before it runs on `stream`.
+svo_status sync_staged_normals(const StagedLaunch& sg, void* stream);
 svo_status stage_views(const svo_view* views, uint32_t n, int policy, const LaunchShape& shape, int device,
                        StagedLaunch* out);
 LaunchShape renderer_shape(const svo_renderer* r);

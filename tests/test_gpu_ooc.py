@@ -165,3 +165,41 @@ def test_fetched_readback_states(terrain8):
     st = mm.updateClientSideIncoreBuffer(s1)
     assert st["requests"] == 0 and st["inserted"] == 0
     assert mm.info["resident_treelets"] == 1
+
+
+def test_normals_follow_updates_into_shading(terrain8):
+    """Uploads leave the normals of refilled slots for the first launch that
+    shades with them: phong and normal views after out-of-core updates match
+    the oracle over the manager's own buffers, and the device buffer equals
+    the host's once read back."""
+    sc, _ = terrain8
+    tb = sc.info["treelet_bytes"]
+    mm = svo.TreeletMemoryManagerHip(sc, 48 * tb, device=0)
+    s = svo.Stream()
+    cams = _path(6, 200, 150, lod=0.25)
+    for f, cam in enumerate(cams):
+        mm.performAnalysePass(cam, 4, (0.0, 0.0), "treelet", s)
+        mm.updateClientSideIncoreBuffer(s)
+        mm.updateDeviceMemory(s)
+        mode = "phong" if f % 2 else "normal"
+        _say(f"frame {f}: {mode} after {mm.info['resident_treelets']} resident")
+        vcam = svo.with_view_mode(cam, mode)
+        g = svo.render(mm, vcam, "treelet", shade=True)
+        n = mm.info["pool_nodes"]
+        nodes, attr = mm.getIncoreBuffer(0, n)
+        r = _oracle_shade(mm, vcam, nodes, attr)
+        assert np.array_equal(g["node"], r["node"]), f"frame {f}"
+        if mode == "normal":
+            same = (g["shade"].view(np.uint32) == r["shade"].view(np.uint32)) | (np.isnan(g["shade"]) & np.isnan(r["shade"]))
+            assert same.all(), f"frame {f}: {int((~same).sum())} values differ"
+        else:
+            assert np.allclose(g["shade"], r["shade"], rtol=0, atol=2e-6), f"frame {f}"
+    hn, ha = mm.getIncoreBuffer(0, mm.info["pool_nodes"])
+    dn, da = mm.getDeviceIncoreBuffer(0, mm.info["pool_nodes"])
+    assert np.array_equal(hn, dn) and np.array_equal(ha, da)
+
+
+def _oracle_shade(mm, cam, nodes, attr):
+    import oracle
+    return oracle.render_buffers(nodes, attr[:, 1], np.asarray(mm.getSlots(), np.int32), mm.getNumNodesPerTreelet(),
+                                 mm.scene.info["max_depth"], cam, "treelet", normals=attr[:, 0])
