@@ -573,28 +573,31 @@ struct svo_mm {
   }
 
   // Sends the stale normals (see nrm_stale) in one upload-kernel launch on
-  // `st` and waits for it.
+  // `st` and waits for it; the slots count as current once it has ended.
   svo_status sync_normals(hipStream_t st) {
     if (nrm_slots.empty() || host_only()) return SVO_OK;
     if (nrm_slots.size() + 1 > span_cap) { set_error("normals upload: %zu spans", nrm_slots.size()); return SVO_ERR_STATE; }
     uint32_t n = 0;
     uint64_t chunks = 0;
     for (const uint32_t sl : nrm_slots) {
-      nrm_stale[sl] = 0;
       const size_t n0 = (size_t)sl * N, nn = std::max(extent[sl], dev_nrm_extent[sl]);
-      dev_nrm_extent[sl] = extent[sl];
       if (nn == 0) continue;
       h_spans[n++] = UploadSpan{(uint64_t)(uintptr_t)(normals.data() + n0), (uint64_t)(uintptr_t)(d_normals + n0), nn * 4,
                                 chunks};
       chunks += (nn * 4 + 15) / 16;
     }
+    if (n != 0) {
+      HIP_TRY(hipSetDevice(device));
+      HIP_TRY(hipMemcpyAsync(d_spans, h_spans, (size_t)n * sizeof(UploadSpan), hipMemcpyHostToDevice, st));
+      const int e = launch_upload(d_spans, n, chunks, st);
+      if (e != 0) { set_error("normals upload launch failed: %s", hipGetErrorString((hipError_t)e)); return SVO_ERR_HIP; }
+      HIP_TRY(hipStreamSynchronize(st));
+    }
+    for (const uint32_t sl : nrm_slots) {
+      nrm_stale[sl] = 0;
+      dev_nrm_extent[sl] = extent[sl];
+    }
     nrm_slots.clear();
-    if (n == 0) return SVO_OK;
-    HIP_TRY(hipSetDevice(device));
-    HIP_TRY(hipMemcpyAsync(d_spans, h_spans, (size_t)n * sizeof(UploadSpan), hipMemcpyHostToDevice, st));
-    const int e = launch_upload(d_spans, n, chunks, st);
-    if (e != 0) { set_error("normals upload launch failed: %s", hipGetErrorString((hipError_t)e)); return SVO_ERR_HIP; }
-    HIP_TRY(hipStreamSynchronize(st));
     return SVO_OK;
   }
 
