@@ -542,11 +542,6 @@ struct svo_mm {
       for (uint32_t sl = r.first; sl < r.first + r.second; ++sl) {
         if (!refilled[sl]) continue;
         const size_t n0 = (size_t)sl * N, nn = std::max(extent[sl], dev_extent[sl]);
-        dev_extent[sl] = extent[sl];
-        if (!nrm_stale[sl]) {
-          nrm_stale[sl] = 1;
-          nrm_slots.push_back(sl);
-        }
         if (nn == 0) continue;
         add(incore.data() + n0, d_nodes + n0, nn * 8);
         add(colors.data() + n0, d_colors + n0, nn * 4);
@@ -567,7 +562,17 @@ struct svo_mm {
     const int e = launch_upload(d_spans, n, chunks, st);
     if (e != 0) { set_error("upload launch failed: %s", hipGetErrorString((hipError_t)e)); return SVO_ERR_HIP; }
     HIP_TRY(hipStreamSynchronize(st));
-    for (const auto& r : runs) std::fill(refilled.begin() + r.first, refilled.begin() + r.first + r.second, 0u);
+    // the device now holds what was sent: record it
+    for (const auto& r : runs)
+      for (uint32_t sl = r.first; sl < r.first + r.second; ++sl) {
+        if (!refilled[sl]) continue;
+        refilled[sl] = 0;
+        dev_extent[sl] = extent[sl];
+        if (!nrm_stale[sl]) {
+          nrm_stale[sl] = 1;
+          nrm_slots.push_back(sl);
+        }
+      }
     patched.clear();
     return SVO_OK;
   }
