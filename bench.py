@@ -11,6 +11,10 @@ inputs are resident in HBM before the timed region starts.
 Also reported in the same line:
 * ``policies``: the same frame under both GPU traversal policies (the
   treelet renderer's and the CPU caster's), each with its own tuned shape;
+* ``reference_case``: the reference's own CPU-runnable case (level-4 Menger
+  sponge at depth 8 in one treelet, 512x512 rays from a seeded point on a
+  sphere of radius 2.5) under both GPU policies, next to the oracle's
+  restatement of the CPU caster on the host;
 * ``batched``: 64 views over the depth-11 terrain and a depth-10 sphere
   cloud, resolutions uniform over {256^2, 512^2, 720p, 1080p}, one launch
   captured in a hipGraph, launch shape from the autotuner;
@@ -19,7 +23,7 @@ Also reported in the same line:
   each frame runs the analyse pass, the feedback update (page-ins,
   evictions), the upload and the render: frames/s and page-ins per frame,
   one stage after the other, and pipelined (frame i's render overlapping
-  frame i+1's analyse pass and feedback update);
+  the host update of frame i+1);
 * ``serving``: Poisson request streams over four resident scenes (Menger
   depth 9, terrain depth 11, sphere cloud depth 10, terrain depth 12) in a
   64 GB HBM budget, served by the continuous-batching server and by the plain
@@ -103,6 +107,36 @@ def cpu_baseline(scene, cam, threads, frames=5):
                    f"median of {frames} frames after 1 warm-up on {threads} threads; single thread "
                    f"{rays / single / 1e6:.2f} Mrays/s; treelet policy {rays / pol_a / 1e6:.2f} Mrays/s"),
     }
+
+
+def reference_case_section(stream, threads, cpu=True, launches=50):
+    """The reference's own CPU-runnable case: a level-4 Menger sponge at depth
+    8 in one treelet, 512x512 primary rays from a seeded point on a sphere of
+    radius 2.5 (60 degree field of view), both GPU policies autotuned, next
+    to the oracle's restatement of the CPU caster on the host."""
+    import libsvo_amd as svo
+    from libsvo_amd import workloads
+    sc = workloads.menger_scene(depth=8, level=4, treelet_bytes=0)
+    mm = svo.TreeletMemoryManagerHip(sc, workloads.incore_bytes_for(sc), device=0, resident_all=True)
+    cam = workloads.sphere_camera(workloads.MENGER_CAMERA_SEED, 512, 512)
+    rays = cam.width * cam.height
+    view = [svo.View(mm, cam, svo.Frame.alloc(cam.width, cam.height))]
+    res = {"workload": "Menger sponge level 4, depth 8, one treelet, 512x512 primary rays, LOD off",
+           "tree_nodes": sc.info["tree_nodes"]}
+    for policy in ("treelet", "cpu3"):
+        r = svo.Renderer(0)
+        shape, _ = r.autotune(view, policy, stream, iters=10)
+        r.timed(view, policy, stream, iters=5)
+        ms = r.timed(view, policy, stream, iters=launches)
+        res[policy] = {"ms_per_launch": round(ms, 5), "mrays_per_s": round(rays / ms / 1e3, 1), "launch_shape": list(shape)}
+        log(f"reference case, {policy} policy: {shape} {ms:.4f} ms/launch = {rays / ms / 1e3:.1f} Mrays/s")
+    if cpu:
+        base = cpu_baseline(sc, cam, threads)
+        res["cpu_baseline"] = base
+        res["gpu_over_cpu"] = round(res["cpu3"]["mrays_per_s"] / base["value"], 1)
+        log(f"reference case on the CPU: {base['value']:.2f} Mrays/s on {threads} threads; "
+            f"GPU (same policy) / CPU = {res['gpu_over_cpu']}x")
+    return res
 
 
 def _scene(cache, key):
@@ -347,6 +381,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-ooc", action="store_true")
+    ap.add_argument("--no-reference-case", action="store_true",
+                    help="skip the reference's own case (Menger depth 8 in one treelet, 512x512)")
     ap.add_argument("--ooc-frames", type=int, default=300)
     ap.add_argument("--serving-seconds", type=float, default=3.0, help="length of each request stream")
     ap.add_argument("--serving-rates", default="200,1000,2000", help="Poisson arrival rates (req/s)")
@@ -454,6 +490,9 @@ def main():
         },
     }
     extra_scenes = {"terrain11": (scene, mm)}
+    if not args.no_reference_case:
+        out["reference_case"] = reference_case_section(stream, min(16, os.cpu_count() or 1),
+                                                       cpu=not args.no_cpu_baseline)
     if not args.no_batched:
         out["batched"] = batched_section(extra_scenes, stream, args.policy)
     if not args.no_ooc:
