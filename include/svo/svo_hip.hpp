@@ -11,7 +11,10 @@
 // Differences a caller sees: errors throw svo::HipError (status + message)
 // instead of printing and exit(); device buffers are plain device pointers
 // (svo_frame_out) instead of libbencl gids; cameras are the FP32 uniforms of
-// svo_camera instead of gloost cameras (see INTEGRATION.md for the mapping).
+// svo_camera instead of gloost cameras: the far-plane lower-left corner, the
+// one-pixel steps along the far plane and the eye position, all in model
+// space, plus tScaleRatio and the frame size (the kernel arguments that
+// raycast_serialized_opencl.cpp:729-749 sets).
 #ifndef SVO_HIP_HPP
 #define SVO_HIP_HPP
 
