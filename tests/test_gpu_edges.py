@@ -1,7 +1,8 @@
 """GPU edge cases against the oracle, bit for bit: the deepest scenes each
 policy's 16-level stack allows (a wavy surface patch at depth 15 and 16),
 the depth limit refused cleanly, frames of one pixel, one row and one
-column through every launch shape, and a scene of a single leaf."""
+column through every launch shape, the largest frame sides, and a scene of
+a single leaf."""
 import numpy as np
 import pytest
 
@@ -122,6 +123,35 @@ def test_degenerate_frames_every_shape(menger8):
         svo._capi.check(svo.lib.svo_device_synchronize(), "sync")
         for (w, h), v, ref in zip(sizes, views, refs):
             _same(v.frame.fetch(), ref, f"{w}x{h} shape {shape}")
+
+
+def test_largest_frame_sides(menger8):
+    """The largest frame sides the renderer takes (16384 pixels) as a row of
+    16384 x 3 and a column of 5 x 16384, next to each other in one launch,
+    bit-identical to the oracle; a side of 16385 is refused."""
+    import oracle
+    from libsvo_amd import workloads
+    sc, base = menger8
+    ib = workloads.incore_bytes_for(sc)
+    mm = svo.TreeletMemoryManagerHip(sc, ib, 0, resident_all=True)
+    ora = oracle.OracleScene(base, ib)
+    sizes = [(16384, 3), (5, 16384)]
+    views = [svo.View(mm, workloads.sphere_camera(950 + i, w, h), svo.Frame.alloc(w, h)) for i, (w, h) in enumerate(sizes)]
+    refs = [ora.render(v.camera, "treelet") for v in views]
+    r = svo.Renderer(0)
+    for v in views:
+        v.frame.node.fill(0xAB)
+        v.frame.rgba.fill(0xAB)
+    _say("frames 16384x3 and 5x16384 in one launch")
+    r.render(views, "treelet")
+    svo._capi.check(svo.lib.svo_device_synchronize(), "sync")
+    for (w, h), v, ref in zip(sizes, views, refs):
+        g = v.frame.fetch()
+        _same(g, ref, f"{w}x{h}")
+        assert (g["node"] != svo.MISS).sum() > 0, f"{w}x{h}: no hit"
+    for w, h in [(16385, 1), (1, 16385)]:
+        with pytest.raises(svo.SvoError):
+            r.render([svo.View(mm, workloads.sphere_camera(960, w, h), svo.Frame.alloc(1, 1))], "treelet")
 
 
 def test_single_leaf_scene(tmp_path):
