@@ -1,8 +1,9 @@
 """GPU edge cases against the oracle, bit for bit: the deepest scenes each
 policy's 16-level stack allows (a wavy surface patch at depth 15 and 16),
 the depth limit refused cleanly, frames of one pixel, one row and one
-column through every launch shape, the largest frame sides, and a scene of
-a single leaf."""
+column through every launch shape, the largest frame sides, eyes exactly
+on the octree's planes (signed zeros included), and a scene of a single
+leaf."""
 import numpy as np
 import pytest
 
@@ -152,6 +153,36 @@ def test_largest_frame_sides(menger8):
     for w, h in [(16385, 1), (1, 16385)]:
         with pytest.raises(svo.SvoError):
             r.render([svo.View(mm, workloads.sphere_camera(960, w, h), svo.Frame.alloc(1, 1))], "treelet")
+
+
+@pytest.mark.parametrize("policy", ["treelet", "cpu3"])
+def test_eyes_on_octree_planes(menger8, policy):
+    """Eyes exactly on the octree's planes: on the root's midplanes (0 and
+    the negative zero), on deeper dyadic planes, on a face of the cube and
+    inside it on a node corner.  Plane parameters of such rays are exact
+    zeros (signed as the IEEE rules give them) and entries land exactly on
+    edges, so every select and comparison of the traversal decides ties
+    there; bit-identical to the oracle."""
+    import oracle
+    from libsvo_amd import workloads
+    sc, base = menger8
+    ib = workloads.incore_bytes_for(sc)
+    mm = svo.TreeletMemoryManagerHip(sc, ib, 0, resident_all=True)
+    ora = oracle.OracleScene(base, ib)
+    eyes = [((0.0, 0.0, 2.5), (0.0, 0.0, 0.0)), ((-0.0, 0.5, 2.0), (0.1, 0.05, 0.0)),
+            ((0.0, -0.0, -2.0), (0.0, 0.0, 0.0)), ((0.25, 0.125, 1.0), (-0.2, 0.1, -0.5)),
+            ((1.0, 0.5, 0.25), (-1.0, 0.0, 0.0)), ((0.25, 0.25, 0.25), (0.0, 0.0, 0.0)),
+            ((-0.0, -0.0, -0.0), (0.3, 0.2, 0.1))]
+    r = svo.Renderer(0)
+    for i, (eye, target) in enumerate(eyes):
+        cam = svo.look_at(eye, target, (0.0, 1.0, 0.0), 60.0, 64, 48, far=20.0)
+        f = svo.Frame.alloc(64, 48)
+        f.node.fill(0xAB)
+        f.rgba.fill(0xAB)
+        _say(f"eye {eye} -> {target}, policy={policy}")
+        r.render([svo.View(mm, cam, f)], policy)
+        svo._capi.check(svo.lib.svo_device_synchronize(), "sync")
+        _same(f.fetch(), ora.render(cam, policy), f"eye {eye}")
 
 
 def test_single_leaf_scene(tmp_path):
