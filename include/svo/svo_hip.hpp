@@ -3,7 +3,14 @@
 //
 //   TreeletMemoryManagerHip  <- TreeletMemoryManager + TreeletMemoryManagerCl
 //                               (src/TreeletMemoryManager.h:97-157,
-//                               src/TreeletMemoryManagerCl.h:69-93)
+//                               src/TreeletMemoryManagerCl.h:69-93): getTreelet(s),
+//                               getTreeletSizeInByte, getNumNodesPerTreelet,
+//                               getIncoreBuffer, getAttributeIncoreBuffer,
+//                               updateClientSideIncoreBuffer, insert/remove,
+//                               freeIncorePosition, updateDeviceMemory,
+//                               resetIncoreBuffer, getFreeIncoreSlotStack,
+//                               getIncoreSlotsToUpload, getSlots, initClBuffer,
+//                               the device pools and device
 //   RenderPassAnalyseHip     <- RenderPassAnalyse (src/RenderPassAnalyse.h:72-103)
 //   renderToBuffer()         <- the renderToBuffer enqueue of
 //                               raycast_serialized_opencl.cpp:729-749
@@ -72,11 +79,43 @@ class TreeletMemoryManagerHip {
     return h;
   }
 
+  // getTreelets(): every treelet's header and its slot (Treelet::getSlotGid,
+  // -1 = not resident).
+  struct TreeletState {
+    svo_treelet_header header;
+    int slot;
+  };
+  std::vector<TreeletState> getTreelets() const {
+    const unsigned n = getNumTreelets();
+    std::vector<svo_treelet_header> h(n);
+    std::vector<int32_t> s(n);
+    check(svo_mm_get_treelets(_mm, 0, n, h.data(), s.data()), "getTreelets");
+    std::vector<TreeletState> out(n);
+    for (unsigned i = 0; i < n; ++i) out[i] = TreeletState{h[i], s[i]};
+    return out;
+  }
+
   bool insertTreeletIntoIncoreBuffer(unsigned gid, float error = 0.0f) {
     return svo_mm_insert(_mm, gid, error) == SVO_OK;
   }
   bool removeTreeletFromIncoreBuffer(unsigned gid) { return svo_mm_remove(_mm, gid) == SVO_OK; }
   void resetIncoreBuffer() { check(svo_mm_reset(_mm), "resetIncoreBuffer"); }
+  // freeIncorePosition(): frees an invisible treelet found by the probe rule
+  // and everything resident below it; false when none was found.
+  bool freeIncorePosition(int* freedGid = nullptr) {
+    int32_t g = -1;
+    check(svo_mm_free_incore_position(_mm, &g), "freeIncorePosition");
+    if (freedGid) *freedGid = g;
+    return g >= 0;
+  }
+  // getFreeIncoreSlotStack(): bottom of the stack first (back() is the top).
+  std::vector<unsigned> getFreeIncoreSlotStack() const {
+    return listOf(svo_mm_get_free_slot_stack, "getFreeIncoreSlotStack");
+  }
+  // getIncoreSlotsToUpload(): slots changed since the last upload, ascending.
+  std::vector<unsigned> getIncoreSlotsToUpload() const {
+    return listOf(svo_mm_get_slots_to_upload, "getIncoreSlotsToUpload");
+  }
   void makeAllResident() { check(svo_mm_make_resident_all(_mm), "makeAllResident"); }
 
   // updateClientSideIncoreBuffer(RenderPassAnalyse*): applies the analyse
@@ -101,6 +140,13 @@ class TreeletMemoryManagerHip {
     return out;
   }
 
+  // getSlots() with the whole SlotInfo (gid, visibility, error).
+  std::vector<svo_slot_info> getSlotInfos() const {
+    std::vector<svo_slot_info> out(info().num_slots);
+    check(svo_mm_get_slots(_mm, 0, (uint32_t)out.size(), out.data()), "getSlotInfos");
+    return out;
+  }
+
   // Copy of the host incore buffer (getIncoreBuffer) and its colours and
   // normals (getAttributeIncoreBuffer).
   std::vector<svo_node> getIncoreBuffer(std::vector<uint32_t>* attribs = nullptr) const {
@@ -111,6 +157,41 @@ class TreeletMemoryManagerHip {
     if (attribs) *attribs = std::move(a);
     return nodes;
   }
+  // The host incore buffer itself (the reference returns a reference to its
+  // vector): pool_nodes nodes, valid while the manager lives.
+  const svo_node* getIncoreBufferData(unsigned long long* count = nullptr) const {
+    const svo_node* p = nullptr;
+    uint64_t n = 0;
+    check(svo_mm_get_incore_buffer(_mm, &p, &n), "getIncoreBuffer");
+    if (count) *count = n;
+    return p;
+  }
+  // The attribute incore buffer itself, as its normal and colour arrays.
+  void getAttributeIncoreBuffer(const uint32_t** normals, const uint32_t** colors,
+                                unsigned long long* count = nullptr) const {
+    uint64_t n = 0;
+    check(svo_mm_get_attribute_incore_buffer(_mm, normals, colors, &n), "getAttributeIncoreBuffer");
+    if (count) *count = n;
+  }
+
+  // TreeletMemoryManagerCl's device side: initClBuffer() uploads the whole
+  // incore buffer; getClIncoreBuffer() / getClAttributeIncoreBuffers() give
+  // the HBM pools (device pointers in place of libbencl gids) and
+  // getDevice() the device (in place of getContext()).  Once the pools were
+  // taken, every update also sends the normals.
+  void initClBuffer(void* stream = nullptr) { check(svo_mm_init_device_buffer(_mm, stream), "initClBuffer"); }
+  svo_device_pools getDevicePools() {
+    svo_device_pools p;
+    check(svo_mm_get_device_pools(_mm, &p), "getDevicePools");
+    return p;
+  }
+  const svo_node* getClIncoreBuffer() { return static_cast<const svo_node*>(getDevicePools().nodes); }
+  void getClAttributeIncoreBuffers(const uint32_t** normals, const uint32_t** colors) {
+    const svo_device_pools p = getDevicePools();
+    if (normals) *normals = static_cast<const uint32_t*>(p.normals);
+    if (colors) *colors = static_cast<const uint32_t*>(p.colors);
+  }
+  int getDevice() const { return (int)info().device; }
 
   svo_mm_info info() const {
     svo_mm_info i;
@@ -121,6 +202,14 @@ class TreeletMemoryManagerHip {
   const svo_scene* scene() const { return _scene; }
 
  private:
+  template <typename Fn>
+  std::vector<unsigned> listOf(Fn fn, const char* where) const {
+    uint32_t n = 0;
+    check(fn(_mm, nullptr, 0, &n), where);
+    std::vector<uint32_t> v(n);
+    check(fn(_mm, v.data(), n, &n), where);
+    return std::vector<unsigned>(v.begin(), v.end());
+  }
   svo_scene_info sceneInfo() const {
     svo_scene_info i;
     check(svo_scene_get_info(_scene, &i), "svo_scene_get_info");

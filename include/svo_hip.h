@@ -83,7 +83,11 @@ typedef struct svo_scene_info {
   uint32_t num_treelets;
   uint32_t treelet_bytes;       /* mem_size of every treelet            */
   uint32_t nodes_per_treelet;   /* treelet_bytes / 8                    */
-  uint32_t max_depth;           /* depth of the deepest leaf (root = 0) */
+  uint32_t max_depth;           /* depth of the deepest valid node (root =
+                                   0): the deepest leaf of a well-formed
+                                   tree; an inner node below every leaf
+                                   counts too, so that the traversal stack
+                                   holds every push                      */
   uint64_t total_nodes;         /* sum of num_nodes over treelets       */
   uint64_t tree_nodes;          /* distinct octree nodes                */
   uint64_t tree_leaves;         /* octree leaves at max_depth           */
@@ -313,6 +317,90 @@ svo_status svo_mm_read_device_incore(const svo_mm* mm, uint64_t first,
                                      uint64_t count, svo_node* nodes,
                                      uint32_t* attribs);
 
+/* getTreelets() (src/TreeletMemoryManager.h:107, the manager's
+ * std::vector<Treelet*>): header and slot (Treelet::getSlotGid, -1 = not
+ * resident) of treelets [first, first+count); either array may be NULL. */
+svo_status svo_mm_get_treelets(const svo_mm* mm, uint32_t first, uint32_t count,
+                               svo_treelet_header* headers, int32_t* slots);
+
+/* getIncoreBuffer() (src/TreeletMemoryManager.h:117): the host incore buffer
+ * itself, *count = pool_nodes nodes (node n of slot s at s*N + n), borrowed:
+ * valid until the handle is destroyed, changed by inserts, removes, resets and
+ * feedback updates.  Page-locked when the handle has a device. */
+svo_status svo_mm_get_incore_buffer(const svo_mm* mm, const svo_node** nodes,
+                                    uint64_t* count);
+
+/* getAttributeIncoreBuffer() (src/TreeletMemoryManager.h:121): the attribute
+ * incore buffer, one {packed_normal, packed_color} package per incore node,
+ * held as two borrowed arrays of *count words (same lifetime as
+ * svo_mm_get_incore_buffer); either pointer may be NULL. */
+svo_status svo_mm_get_attribute_incore_buffer(const svo_mm* mm,
+                                              const uint32_t** normals,
+                                              const uint32_t** colors,
+                                              uint64_t* count);
+
+/* freeIncorePosition() (src/TreeletMemoryManager.h:135,
+ * src/TreeletMemoryManager.cpp:560-653): probes up to 100 slots from the
+ * manager's cursor for a treelet with visibility < 0 whose parent also has
+ * visibility < 0 and frees it and every resident treelet below it.
+ * *freed_gid = the treelet freed, -1 if none was found (the reference
+ * returns false). */
+svo_status svo_mm_free_incore_position(svo_mm* mm, int32_t* freed_gid);
+
+/* getFreeIncoreSlotStack() (src/TreeletMemoryManager.h:150): the free slots,
+ * bottom of the stack first; slots[n-1] is the top, the slot the next insert
+ * takes.  Up to `cap` entries are written, *n receives the full count. */
+svo_status svo_mm_get_free_slot_stack(const svo_mm* mm, uint32_t* slots,
+                                      uint32_t cap, uint32_t* n);
+
+/* getIncoreSlotsToUpload() (src/TreeletMemoryManager.h:153): the slots
+ * changed since the last svo_mm_update_device, ascending.  Up to `cap`
+ * entries are written, *n receives the full count. */
+svo_status svo_mm_get_slots_to_upload(const svo_mm* mm, uint32_t* slots,
+                                      uint32_t cap, uint32_t* n);
+
+/* getSlots() (src/TreeletMemoryManager.h:157) with the whole SlotInfo
+ * (:72-89): treelet gid (-1 empty), visibility, error of slots
+ * [first, first+count). */
+typedef struct svo_slot_info {
+  int32_t treelet_gid;
+  int32_t visibility;
+  float error;
+  uint32_t reserved;
+} svo_slot_info;
+svo_status svo_mm_get_slots(const svo_mm* mm, uint32_t first, uint32_t count,
+                            svo_slot_info* out);
+
+/* The HBM pools of the handle, for kernels of the caller's own that read the
+ * incore buffer, as the OpenCL app binds getClIncoreBufferGid() and
+ * getClAttributeIncoreBufferGid() (src/TreeletMemoryManagerCl.h:76-90,
+ * raycast_serialized_opencl.cpp:323-324); `device` stands for getContext().
+ *   nodes    : pool_nodes x 8 B {first_child, masks}
+ *   colors   : pool_nodes x u32 packed_color   } the reference's interleaved
+ *   normals  : pool_nodes x u32 packed_normal  } Attribs, as two arrays
+ *   slot_gid : num_slots x i32, gid per slot (-1 empty)
+ * Owned by the handle.  The pools equal the host incore buffer once
+ * svo_mm_update_device has run after the last change; this call sends any
+ * normals still pending and, from then on, every update sends normals with
+ * nodes and colours. */
+typedef struct svo_device_pools {
+  void* nodes;
+  void* colors;
+  void* normals;
+  void* slot_gid;
+  uint64_t pool_nodes;
+  uint32_t num_slots;
+  uint32_t nodes_per_treelet;
+  int32_t device;
+  uint32_t reserved;
+} svo_device_pools;
+svo_status svo_mm_get_device_pools(svo_mm* mm, svo_device_pools* out);
+
+/* initClBuffer() (src/TreeletMemoryManagerCl.cpp:101-113): uploads the whole
+ * incore buffer (nodes and attributes) to the HBM pools on `stream` and waits
+ * for it.  svo_mm_create already did this once. */
+svo_status svo_mm_init_device_buffer(svo_mm* mm, void* stream);
+
 /* ------------------------------------------------------------ rendering */
 
 /* Traversal policy.  SVO_POLICY_TREELET is the OpenCL renderer's
@@ -412,8 +500,14 @@ svo_status svo_autotune(svo_renderer* r, const svo_view* views, uint32_t n,
 /* A batch is a multi-view launch captured once into a hipGraph and replayed
  * with svo_batch_launch.  Its view descriptors live in device memory owned by
  * the batch; svo_batch_set_cameras rewrites the camera uniforms (same frame
- * sizes) with an asynchronous copy on `stream`, ordered before the next
- * launch on that stream. */
+ * sizes; view modes are checked, and the normals of handles whose views now
+ * shade with them are sent before the next replay) with an asynchronous copy
+ * on `stream`, ordered before the next launch on that stream.  Views with
+ * feedback_stride > 0 record into the analysis cycle that is current on the
+ * device when the replay runs (the readback of a cycle advances it on the
+ * stream), so a batch keeps feeding out-of-core updates across cycles.  A
+ * replay is refused while a view's handle has slots waiting for
+ * svo_mm_update_device, like svo_render. */
 typedef struct svo_batch svo_batch;
 svo_status svo_batch_create(svo_renderer* r, const svo_view* views, uint32_t n,
                             int policy, void* stream, svo_batch** out);

@@ -354,6 +354,94 @@ class TreeletMemoryManagerHip:
               "svo_mm_read_host_incore")
         return nodes, attr
 
+    def getIncoreBufferView(self) -> np.ndarray:
+        """The host incore buffer itself (getIncoreBuffer), as a zero-copy
+        (pool_nodes, 2) uint32 view {first_child, masks}; valid while this
+        manager lives, changed by its updates."""
+        p, n = ctypes.c_void_p(), ctypes.c_uint64()
+        check(lib.svo_mm_get_incore_buffer(self._h, ctypes.byref(p), ctypes.byref(n)), "svo_mm_get_incore_buffer")
+        return np.ctypeslib.as_array(ctypes.cast(p, ctypes.POINTER(ctypes.c_uint32)), (n.value, 2))
+
+    def getAttributeIncoreBufferView(self) -> Tuple[np.ndarray, np.ndarray]:
+        """The attribute incore buffer itself (getAttributeIncoreBuffer):
+        zero-copy uint32 views (normals, colors), one word per incore node."""
+        pn, pc, n = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_uint64()
+        check(lib.svo_mm_get_attribute_incore_buffer(self._h, ctypes.byref(pn), ctypes.byref(pc), ctypes.byref(n)),
+              "svo_mm_get_attribute_incore_buffer")
+        u = ctypes.POINTER(ctypes.c_uint32)
+        return (np.ctypeslib.as_array(ctypes.cast(pn, u), (n.value,)),
+                np.ctypeslib.as_array(ctypes.cast(pc, u), (n.value,)))
+
+    def getAttributeIncoreBuffer(self, first: int = 0, count: Optional[int] = None) -> np.ndarray:
+        """(n, 2) uint32 copy of the attribute packages {packed_normal,
+        packed_color} of incore nodes [first, first+count)."""
+        nrm, col = self.getAttributeIncoreBufferView()
+        end = nrm.size if count is None else first + count
+        return np.stack([nrm[first:end], col[first:end]], axis=1)
+
+    def getTreelets(self) -> Tuple[np.ndarray, np.ndarray]:
+        """getTreelets(): ((num_treelets, 8) uint32 headers in file order,
+        (num_treelets,) int32 slot of each treelet, -1 = not resident)."""
+        n = self.scene.info["num_treelets"]
+        hdr = (_capi.svo_treelet_header * n)()
+        slots = np.empty(n, np.int32)
+        check(lib.svo_mm_get_treelets(self._h, 0, n, hdr, slots.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))),
+              "svo_mm_get_treelets")
+        h = np.ctypeslib.as_array(ctypes.cast(hdr, ctypes.POINTER(ctypes.c_uint32)), (n, 8)).copy()
+        return h, slots
+
+    def getTreelet(self, gid: int) -> Dict[str, int]:
+        """Header of treelet `gid` plus its slot (-1 = not resident)."""
+        hdr = _capi.svo_treelet_header()
+        slot = ctypes.c_int32()
+        check(lib.svo_mm_get_treelets(self._h, gid, 1, ctypes.byref(hdr), ctypes.byref(slot)), "svo_mm_get_treelets")
+        out = {k: getattr(hdr, k) for k, _ in _capi.svo_treelet_header._fields_}
+        out["slot"] = slot.value
+        return out
+
+    def freeIncorePosition(self) -> int:
+        """freeIncorePosition(): frees one invisible treelet (and everything
+        resident below it) found by the reference's probe rule; returns its
+        gid, -1 if none was found."""
+        g = ctypes.c_int32()
+        check(lib.svo_mm_free_incore_position(self._h, ctypes.byref(g)), "svo_mm_free_incore_position")
+        return g.value
+
+    def _u32_list(self, fn, name) -> List[int]:
+        n = ctypes.c_uint32()
+        check(fn(self._h, None, 0, ctypes.byref(n)), name)
+        buf = (ctypes.c_uint32 * max(1, n.value))()
+        check(fn(self._h, buf, n.value, ctypes.byref(n)), name)
+        return list(buf[:n.value])
+
+    def getFreeIncoreSlotStack(self) -> List[int]:
+        """Free slots, bottom of the stack first (the last one is taken next)."""
+        return self._u32_list(lib.svo_mm_get_free_slot_stack, "svo_mm_get_free_slot_stack")
+
+    def getIncoreSlotsToUpload(self) -> List[int]:
+        """Slots changed since the last updateDeviceMemory, ascending."""
+        return self._u32_list(lib.svo_mm_get_slots_to_upload, "svo_mm_get_slots_to_upload")
+
+    def getSlotInfos(self) -> List[Tuple[int, int, float]]:
+        """getSlots() with the whole SlotInfo: (treelet gid, visibility,
+        error) per slot."""
+        n = self.info["num_slots"]
+        buf = (_capi.svo_slot_info * n)()
+        check(lib.svo_mm_get_slots(self._h, 0, n, buf), "svo_mm_get_slots")
+        return [(b.treelet_gid, b.visibility, b.error) for b in buf]
+
+    def getDevicePools(self) -> Dict[str, int]:
+        """Device pointers of the HBM pools (getClIncoreBufferGid /
+        getClAttributeIncoreBufferGid); from this call on every update also
+        sends the normals."""
+        p = _capi.svo_device_pools()
+        check(lib.svo_mm_get_device_pools(self._h, ctypes.byref(p)), "svo_mm_get_device_pools")
+        return {k: getattr(p, k) for k, _ in _capi.svo_device_pools._fields_ if k != "reserved"}
+
+    def initDeviceBuffer(self, stream: Optional["Stream"] = None) -> None:
+        """initClBuffer(): uploads the whole incore buffer again."""
+        check(lib.svo_mm_init_device_buffer(self._h, stream.handle if stream else None), "svo_mm_init_device_buffer")
+
     def getDeviceIncoreBuffer(self, first: int = 0, count: Optional[int] = None):
         n = self.info["pool_nodes"] - first if count is None else count
         nodes = np.empty((n, 2), np.uint32)

@@ -99,6 +99,15 @@ class svo_server_stats(ctypes.Structure):
                 ("rejected", u64)]
 
 
+class svo_slot_info(ctypes.Structure):
+    _fields_ = [("treelet_gid", i32), ("visibility", i32), ("error", f32), ("reserved", u32)]
+
+
+class svo_device_pools(ctypes.Structure):
+    _fields_ = [("nodes", vp), ("colors", vp), ("normals", vp), ("slot_gid", vp), ("pool_nodes", u64),
+                ("num_slots", u32), ("nodes_per_treelet", u32), ("device", i32), ("reserved", u32)]
+
+
 class svo_launch_config(ctypes.Structure):
     _fields_ = [("tile_width", u32), ("waves_per_group", u32), ("stack_layout", u32), ("reserved", u32)]
 
@@ -131,6 +140,15 @@ SIGNATURES = {
     "svo_mm_treelet_slot": (ctypes.c_int, [vp, u32, P(i32)]),
     "svo_mm_read_host_incore": (ctypes.c_int, [vp, u64, u64, vp, vp]),
     "svo_mm_read_device_incore": (ctypes.c_int, [vp, u64, u64, vp, vp]),
+    "svo_mm_get_treelets": (ctypes.c_int, [vp, u32, u32, P(svo_treelet_header), P(i32)]),
+    "svo_mm_get_incore_buffer": (ctypes.c_int, [vp, P(vp), P(u64)]),
+    "svo_mm_get_attribute_incore_buffer": (ctypes.c_int, [vp, P(vp), P(vp), P(u64)]),
+    "svo_mm_free_incore_position": (ctypes.c_int, [vp, P(i32)]),
+    "svo_mm_get_free_slot_stack": (ctypes.c_int, [vp, P(u32), u32, P(u32)]),
+    "svo_mm_get_slots_to_upload": (ctypes.c_int, [vp, P(u32), u32, P(u32)]),
+    "svo_mm_get_slots": (ctypes.c_int, [vp, u32, u32, P(svo_slot_info)]),
+    "svo_mm_get_device_pools": (ctypes.c_int, [vp, P(svo_device_pools)]),
+    "svo_mm_init_device_buffer": (ctypes.c_int, [vp, vp]),
     "svo_render": (ctypes.c_int, [vp, P(svo_camera), u32, P(svo_frame_out), ctypes.c_int, vp]),
     "svo_renderer_create": (ctypes.c_int, [ctypes.c_int, P(vp)]),
     "svo_renderer_destroy": (None, [vp]),

@@ -114,6 +114,10 @@ struct svo_mm {
   std::vector<uint32_t> dev_nrm_extent;
   std::vector<uint8_t> nrm_stale;
   std::vector<uint32_t> nrm_slots;
+  // Set once a caller has taken the device pools (svo_mm_get_device_pools):
+  // from then on every update sends the normals too, since kernels outside
+  // this library may read them.
+  bool eager_normals = false;
   // What the next upload must send: slots whose content was replaced (a
   // fill or a reset) go up within their extents; otherwise only the nodes
   // a stitch or unstitch changed (global node indices).
@@ -129,13 +133,13 @@ struct svo_mm {
   unsigned long long* d_fb_slot = nullptr;
   unsigned long long* d_fb_gid = nullptr;
   uint32_t* d_fb_queue = nullptr;
-  uint32_t* d_fb_count = nullptr;
+  uint32_t* d_fb_state = nullptr;            // [0] queue count, [1] the cycle's stamp
   unsigned long long* h_fb_slot = nullptr;
   unsigned long long* h_fb_gid = nullptr;  // key of each asked gid, by queue position
   uint32_t* h_zero = nullptr;               // a pinned 0 that resets the queue count
   uint32_t* h_fb_queue = nullptr;
   uint32_t* h_fb_count = nullptr;
-  uint32_t fb_stamp = 1;
+  uint32_t fb_stamp = 1;  // host copy of the device stamp of the current cycle
   uint64_t evictions = 0;  // treelets freed by freeIncorePosition (incl. descendants)
   // uploads: span descriptors (pinned, and their device copy read by the
   // upload kernel) and the pinned slot table
@@ -172,7 +176,7 @@ struct svo_mm {
     hipFree(d_fb_slot);
     hipFree(d_fb_gid);
     hipFree(d_fb_queue);
-    hipFree(d_fb_count);
+    hipFree(d_fb_state);
     if (h_fb_slot) hipHostFree(h_fb_slot);
     if (h_fb_gid) hipHostFree(h_fb_gid);
     if (h_zero) hipHostFree(h_zero);
@@ -248,7 +252,8 @@ struct svo_mm {
   // freeIncorePosition (src/TreeletMemoryManager.cpp:560-653): probes up to
   // 100 slots from the cursor for an invisible treelet whose parent is
   // invisible too, then frees it and everything resident below it.
-  bool free_position() {
+  // Returns the treelet freed (its descendants went with it), 0 if none.
+  uint32_t free_position() {
     uint32_t freeable = 0;
     uint32_t tests = 0;
     const uint32_t probes = 100;
@@ -264,11 +269,11 @@ struct svo_mm {
       if (search_pos == num_slots) search_pos = 1;
       if (tests == probes) break;
     }
-    if (freeable == 0) return false;
+    if (freeable == 0) return 0;
     free_descendants(freeable);
     remove_one(freeable);
     ++evictions;
-    return true;
+    return freeable;
   }
 
   // The host half of the analyse pass and the client-side update, from
@@ -635,6 +640,7 @@ struct svo_mm {
       svo_status r = upload_runs(runs, st, &s.bytes, &s.bytes_moved);
       if (r != SVO_OK) return r;
       dirty.clear();
+      if (eager_normals && (r = sync_normals(st)) != SVO_OK) return r;
     }
     if (stats) *stats = s;
     return SVO_OK;
@@ -690,7 +696,7 @@ extern "C" svo_status svo_mm_create(const svo_scene* scene, uint64_t incore_byte
   const size_t ng = scene->treelets.size();
   if (hipMalloc(&mm->d_fb_slot, slots * 8) != hipSuccess || hipMalloc(&mm->d_fb_gid, ng * 8) != hipSuccess ||
       hipMalloc(&mm->d_fb_queue, ng * 4) != hipSuccess ||
-      hipMalloc(&mm->d_fb_count, 4) != hipSuccess || hipHostMalloc(&mm->h_fb_slot, slots * 8, 0) != hipSuccess ||
+      hipMalloc(&mm->d_fb_state, 8) != hipSuccess || hipHostMalloc(&mm->h_fb_slot, slots * 8, 0) != hipSuccess ||
       hipHostMalloc(&mm->h_fb_gid, ng * 8, 0) != hipSuccess || hipHostMalloc(&mm->h_fb_queue, ng * 4, 0) != hipSuccess ||
       hipHostMalloc(&mm->h_zero, 4, 0) != hipSuccess ||
       hipHostMalloc(&mm->h_fb_count, 4, 0) != hipSuccess) {
@@ -699,7 +705,10 @@ extern "C" svo_status svo_mm_create(const svo_scene* scene, uint64_t incore_byte
   }
   HIP_TRY(hipMemset(mm->d_fb_slot, 0, slots * 8));
   HIP_TRY(hipMemset(mm->d_fb_gid, 0, ng * 8));
-  HIP_TRY(hipMemset(mm->d_fb_count, 0, 4));
+  {
+    const uint32_t state[2] = {0u, mm->fb_stamp};
+    HIP_TRY(hipMemcpy(mm->d_fb_state, state, sizeof state, hipMemcpyHostToDevice));
+  }
   HIP_TRY(hipEventCreateWithFlags(&mm->fb_event, hipEventDisableTiming));
   *mm->h_zero = 0u;
   mm->span_cap = (uint32_t)(3 * slots + 1);  // three spans per slot, plus the slot table
@@ -778,7 +787,7 @@ extern "C" svo_status svo_mm_get_info(const svo_mm* mm, svo_mm_info* out) {
   out->pool_nodes = (uint64_t)mm->num_slots * mm->N;
   out->device_bytes = mm->host_only() ? 0
                                       : out->pool_nodes * 16 + (uint64_t)mm->num_slots * 12 +
-                                            (uint64_t)mm->scene->treelets.size() * 20 + 4;
+                                            (uint64_t)mm->scene->treelets.size() * 20 + 8;
   return SVO_OK;
 }
 
@@ -829,11 +838,106 @@ extern "C" svo_status svo_mm_read_device_incore(const svo_mm* mm, uint64_t first
   return SVO_OK;
 }
 
+extern "C" svo_status svo_mm_get_treelets(const svo_mm* mm, uint32_t first, uint32_t count, svo_treelet_header* headers,
+                                          int32_t* slots) {
+  if (!mm) { set_error("svo_mm_get_treelets: null handle"); return SVO_ERR_ARG; }
+  const size_t nt = mm->scene->treelets.size();
+  if (first > nt || count > nt - first) { set_error("svo_mm_get_treelets: treelets [%u, %u+%u) out of range (%zu)", first, first, count, nt); return SVO_ERR_ARG; }
+  for (uint32_t i = 0; i < count; ++i) {
+    if (headers) headers[i] = mm->scene->treelets[first + i].h;
+    if (slots) slots[i] = mm->treelet_slot[first + i];
+  }
+  return SVO_OK;
+}
+
+extern "C" svo_status svo_mm_get_incore_buffer(const svo_mm* mm, const svo_node** nodes, uint64_t* count) {
+  if (!mm || !nodes) { set_error("svo_mm_get_incore_buffer: null argument"); return SVO_ERR_ARG; }
+  *nodes = mm->incore.data();
+  if (count) *count = mm->incore.size();
+  return SVO_OK;
+}
+
+extern "C" svo_status svo_mm_get_attribute_incore_buffer(const svo_mm* mm, const uint32_t** normals,
+                                                         const uint32_t** colors, uint64_t* count) {
+  if (!mm || (!normals && !colors)) { set_error("svo_mm_get_attribute_incore_buffer: null argument"); return SVO_ERR_ARG; }
+  if (normals) *normals = mm->normals.data();
+  if (colors) *colors = mm->colors.data();
+  if (count) *count = mm->colors.size();
+  return SVO_OK;
+}
+
+extern "C" svo_status svo_mm_free_incore_position(svo_mm* mm, int32_t* freed_gid) {
+  if (!mm || !freed_gid) { set_error("svo_mm_free_incore_position: null argument"); return SVO_ERR_ARG; }
+  const uint32_t g = mm->free_position();
+  *freed_gid = g == 0 ? -1 : (int32_t)g;
+  return SVO_OK;
+}
+
+extern "C" svo_status svo_mm_get_free_slot_stack(const svo_mm* mm, uint32_t* slots, uint32_t cap, uint32_t* n) {
+  if (!mm || !n || (cap && !slots)) { set_error("svo_mm_get_free_slot_stack: null argument"); return SVO_ERR_ARG; }
+  *n = (uint32_t)mm->free_slots.size();
+  for (uint32_t i = 0; i < *n && i < cap; ++i) slots[i] = mm->free_slots[i];
+  return SVO_OK;
+}
+
+extern "C" svo_status svo_mm_get_slots_to_upload(const svo_mm* mm, uint32_t* slots, uint32_t cap, uint32_t* n) {
+  if (!mm || !n || (cap && !slots)) { set_error("svo_mm_get_slots_to_upload: null argument"); return SVO_ERR_ARG; }
+  *n = (uint32_t)mm->dirty.size();
+  uint32_t i = 0;
+  for (const uint32_t s : mm->dirty) {
+    if (i >= cap) break;
+    slots[i++] = s;
+  }
+  return SVO_OK;
+}
+
+extern "C" svo_status svo_mm_get_slots(const svo_mm* mm, uint32_t first, uint32_t count, svo_slot_info* out) {
+  if (!mm || (count && !out)) { set_error("svo_mm_get_slots: null argument"); return SVO_ERR_ARG; }
+  if (first > mm->num_slots || count > mm->num_slots - first) { set_error("svo_mm_get_slots: slots [%u, %u+%u) out of range (%u)", first, first, count, mm->num_slots); return SVO_ERR_ARG; }
+  for (uint32_t i = 0; i < count; ++i) {
+    const SlotInfo& si = mm->slots[first + i];
+    out[i] = svo_slot_info{si.gid, si.visibility, si.error, 0u};
+  }
+  return SVO_OK;
+}
+
+extern "C" svo_status svo_mm_get_device_pools(svo_mm* mm, svo_device_pools* out) {
+  if (!mm || !out) { set_error("svo_mm_get_device_pools: null argument"); return SVO_ERR_ARG; }
+  if (mm->host_only()) { set_error("svo_mm_get_device_pools: handle has no device pools"); return SVO_ERR_STATE; }
+  HIP_TRY(hipSetDevice(mm->device));
+  // kernels outside the library may read the normals: bring them up to date
+  // now and send them with every later update
+  mm->eager_normals = true;
+  const svo_status r = mm->sync_normals(nullptr);
+  if (r != SVO_OK) return r;
+  *out = svo_device_pools{mm->d_nodes, mm->d_colors, mm->d_normals, mm->d_slot_gid, (uint64_t)mm->num_slots * mm->N,
+                          mm->num_slots, mm->N, mm->device, 0u};
+  return SVO_OK;
+}
+
+extern "C" svo_status svo_mm_init_device_buffer(svo_mm* mm, void* stream) {
+  if (!mm) { set_error("svo_mm_init_device_buffer: null handle"); return SVO_ERR_ARG; }
+  if (mm->host_only()) { set_error("svo_mm_init_device_buffer: handle has no device pools"); return SVO_ERR_STATE; }
+  HIP_TRY(hipSetDevice(mm->device));
+  // every slot goes up again within its extents; the rest of the pools is
+  // zero on both sides already
+  for (uint32_t s = 0; s < mm->num_slots; ++s) {
+    mm->refilled[s] = 1u;
+    mm->mark(s);
+  }
+  svo_upload_stats st;
+  svo_status r = mm->update_device((hipStream_t)stream, &st);
+  if (r != SVO_OK) return r;
+  return mm->sync_normals((hipStream_t)stream);
+}
+
 // ------------------------------------------------------------ render hooks
 
 namespace svo_impl {
 
 int mm_device(const svo_mm* mm) { return mm->device; }
+
+uint32_t mm_dirty_slots(const svo_mm* mm) { return (uint32_t)mm->dirty.size(); }
 
 svo_status mm_sync_normals(svo_mm* mm, void* stream) { return mm->sync_normals((hipStream_t)stream); }
 
@@ -862,9 +966,8 @@ svo_status mm_trace_pools(const svo_mm* mm, int policy, TracePools* P) {
   P->fb_slot = mm->d_fb_slot;
   P->fb_gid = mm->d_fb_gid;
   P->fb_queue = mm->d_fb_queue;
-  P->fb_count = mm->d_fb_count;
+  P->fb_state = mm->d_fb_state;
   P->fb_cap = P->num_gids;
-  P->fb_stamp = mm->fb_stamp;
   if (policy == SVO_POLICY_TREELET) {
     // MAX_STACK_SIZE 13 / MAX_SVO_RAYCAST_DEPTH 12 (treeletRenderer_rgba.cl:7-8),
     // widened to max(13, D+1) so that leaves deeper than 12 are reachable.
@@ -1014,14 +1117,16 @@ namespace {
 svo_status enqueue_export(svo_mm* mm, hipStream_t st) {
   HIP_TRY(hipSetDevice(mm->device));
   const uint32_t ng = (uint32_t)mm->scene->treelets.size();
-  const FeedbackExport x{mm->d_fb_slot, mm->d_fb_count, mm->d_fb_queue, mm->d_fb_gid, mm->num_slots, ng,
-                         mm->h_fb_slot, mm->h_fb_count, mm->h_fb_queue, mm->h_fb_gid};
+  // the kernel reads this cycle and moves the device stamp on: records of
+  // launches ordered after it carry the next stamp
+  const FeedbackExport x{mm->d_fb_slot, mm->d_fb_state, mm->d_fb_queue, mm->d_fb_gid, mm->num_slots, ng,
+                         mm->h_fb_slot, mm->h_fb_count, mm->h_fb_queue, mm->h_fb_gid, mm->fb_stamp + 1u, 0u};
   const int e = launch_feedback_export(x, st);
   if (e != 0) { set_error("feedback readback launch failed: %s", hipGetErrorString((hipError_t)e)); return SVO_ERR_HIP; }
-  HIP_TRY(hipMemcpyAsync(mm->d_fb_count, mm->h_zero, 4, hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(mm->d_fb_state, mm->h_zero, 4, hipMemcpyHostToDevice, st));
   HIP_TRY(hipEventRecord(mm->fb_event, st));
   mm->fb_export_stamp = mm->fb_stamp;
-  ++mm->fb_stamp;  // records of this cycle are now stale
+  ++mm->fb_stamp;
   mm->fb_pending = true;
   return SVO_OK;
 }

@@ -272,6 +272,7 @@ struct svo_batch {
   int policy = 0;
   LaunchShape shape{};
   StagedLaunch sg;
+  std::vector<svo_mm*> mms;  // residency handle of each view
   TraceFrame* d = nullptr;
   TraceFrame* h = nullptr;  // pinned
   hipEvent_t copied = nullptr;
@@ -305,6 +306,8 @@ extern "C" svo_status svo_batch_create(svo_renderer* r, const svo_view* views, u
   b->shape = r->shape;
   svo_status st = stage_views(views, n, policy, b->shape, r->device, &b->sg);
   if (st != SVO_OK) return st;
+  b->mms.resize(n);
+  for (uint32_t i = 0; i < n; ++i) b->mms[i] = views[i].mm;
   const size_t bytes = n * sizeof(TraceFrame);
   HIP_TRY(hipMalloc(&b->d, bytes));
   HIP_TRY(hipHostMalloc(&b->h, bytes, hipHostMallocDefault));
@@ -331,17 +334,28 @@ extern "C" svo_status svo_batch_create(svo_renderer* r, const svo_view* views, u
 extern "C" svo_status svo_batch_set_cameras(svo_batch* b, const svo_camera* cams, uint32_t n, void* stream) {
   if (!b || !cams) { set_error("svo_batch_set_cameras: null argument"); return SVO_ERR_ARG; }
   if (n != b->sg.frames.size()) { set_error("batch has %zu views, got %u cameras", b->sg.frames.size(), n); return SVO_ERR_ARG; }
-  for (uint32_t i = 0; i < n; ++i)
+  for (uint32_t i = 0; i < n; ++i) {
     if (cams[i].width != b->sg.frames[i].cam.width || cams[i].height != b->sg.frames[i].cam.height) {
       set_error("camera %u changes the frame size of the batch", i);
       return SVO_ERR_ARG;
     }
+    if (cams[i].view_mode > SVO_VIEW_QUALITY) { set_error("camera %u: unknown view mode %u", i, cams[i].view_mode); return SVO_ERR_ARG; }
+  }
   HIP_TRY(hipSetDevice(b->device));
   if (b->pending) {
     HIP_TRY(hipEventSynchronize(b->copied));
     b->pending = false;
   }
-  for (uint32_t i = 0; i < n; ++i) b->h[i].cam = cams[i];
+  // the handles whose normals the next replays shade with follow the new
+  // view modes
+  b->sg.normal_handles.clear();
+  for (uint32_t i = 0; i < n; ++i) {
+    b->h[i].cam = cams[i];
+    b->sg.frames[i].cam = cams[i];
+    if (b->sg.frames[i].out.shade && (cams[i].view_mode == SVO_VIEW_NORMAL || cams[i].view_mode == SVO_VIEW_PHONG) &&
+        std::find(b->sg.normal_handles.begin(), b->sg.normal_handles.end(), b->mms[i]) == b->sg.normal_handles.end())
+      b->sg.normal_handles.push_back(b->mms[i]);
+  }
   HIP_TRY(hipMemcpyAsync(b->d, b->h, n * sizeof(TraceFrame), hipMemcpyHostToDevice, (hipStream_t)stream));
   HIP_TRY(hipEventRecord(b->copied, (hipStream_t)stream));
   b->pending = true;
@@ -351,6 +365,11 @@ extern "C" svo_status svo_batch_set_cameras(svo_batch* b, const svo_camera* cams
 extern "C" svo_status svo_batch_launch(svo_batch* b, void* stream) {
   if (!b) { set_error("svo_batch_launch: null batch"); return SVO_ERR_ARG; }
   HIP_TRY(hipSetDevice(b->device));
+  for (svo_mm* mm : b->mms)
+    if (mm_dirty_slots(mm) != 0) {
+      set_error("svo_batch_launch: %u incore slots not uploaded; call svo_mm_update_device first", mm_dirty_slots(mm));
+      return SVO_ERR_STATE;
+    }
   const svo_status ns = sync_staged_normals(b->sg, stream);
   if (ns != SVO_OK) return ns;
   HIP_TRY(hipGraphLaunch(b->exec, (hipStream_t)stream));

@@ -457,7 +457,8 @@ bool read_u32(FILE* f, uint32_t* v) { return fread(v, 4, 1, f) == 1; }
 }  // namespace
 
 // Depth of every node of every treelet, checking that child offsets stay
-// inside the treelet.  Returns the deepest leaf depth, or -1 on a bad offset.
+// inside the treelet.  Returns the depth of the deepest valid node (the
+// deepest leaf in a well-formed tree), or -1 on a bad offset.
 int scene_depth(const svo_scene& sc, std::string* why) {
   const size_t nt = sc.treelets.size();
   std::vector<uint32_t> root_depth(nt, 0);
@@ -479,8 +480,7 @@ int scene_depth(const svo_scene& sc, std::string* why) {
     for (uint32_t i = 0; i < T.h.num_nodes; ++i) {
       const svo_node& n = T.nodes[i];
       const uint32_t vm = (n.masks >> kValidShift) & 255u;
-      const uint32_t lm = n.masks & 255u;
-      if (vm == 0) continue;
+            if (vm == 0) continue;
       for (uint32_t c = 0; c < 8; ++c) {
         if (!(vm & (1u << c))) continue;
         const uint32_t ci = i + nth_child(n.masks, n.first_child, c);
@@ -493,7 +493,9 @@ int scene_depth(const svo_scene& sc, std::string* why) {
           return -1;
         }
         nd[ci] = nd[i] + 1;
-        if ((lm & (1u << c)) && nd[ci] > (uint32_t)maxd) maxd = (int)nd[ci];
+        // every valid node counts, not only leaves: an inner node below the
+        // deepest leaf (a malformed tree) must still fit the traversal stack
+        if (nd[ci] > (uint32_t)maxd) maxd = (int)nd[ci];
       }
     }
     // children treelets of t take their root depth from t's leaf

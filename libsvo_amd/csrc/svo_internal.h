@@ -79,14 +79,17 @@ struct svo_scene {
 
 namespace svo_impl {
 
-// Deepest leaf depth of a scene (root = 0), checking that every child offset
-// stays inside its treelet; -1 (and *why) on a bad offset.  scene.cpp.
+// Depth of the deepest valid node of a scene (root = 0; the deepest leaf in a
+// well-formed tree), checking that every child offset stays inside its
+// treelet; -1 (and *why) on a bad offset.  scene.cpp.
 int scene_depth(const svo_scene& sc, std::string* why);
 
 struct TracePools;
 
 // Residency handles as seen by the renderer (memory_manager.cpp).
 int mm_device(const svo_mm* mm);
+// Slots waiting for svo_mm_update_device.
+uint32_t mm_dirty_slots(const svo_mm* mm);
 svo_renderer* mm_default_renderer(svo_mm* mm);
 // Pools and traversal parameters of `mm` for `policy`; fails while slots
 // wait for upload or when the handle has no device pools.

@@ -30,13 +30,16 @@ struct TracePools {
   // Analysis feedback tables of the handle (renderToFeedbackBuffer fused
   // into the traversal).  Keys are (stamp << 32) | order-preserving bits of
   // the error, combined with atomicMax, so a table needs no clearing between
-  // analysis cycles: entries of older stamps are stale.
+  // analysis cycles: entries of older stamps are stale.  The stamp of the
+  // current cycle lives in device memory (fb_state[1]) and is advanced by the
+  // readback kernel on the stream, so descriptors staged once (a graph
+  // batch) record into whatever cycle is current when they run.
   unsigned long long* fb_slot;   // per slot: the largest key of a record whose hit node lies in it
   unsigned long long* fb_gid;    // per treelet: the largest key of a record asking for it
   uint32_t* fb_queue;            // treelets asked for in this cycle, first-come order
-  uint32_t* fb_count;            // entries appended to fb_queue (may exceed fb_cap)
+  uint32_t* fb_state;            // [0] entries appended to fb_queue (may exceed fb_cap), [1] stamp
   uint32_t fb_cap;
-  uint32_t fb_stamp;
+  uint32_t pad;
 };
 
 // One view of a launch: its scene, camera uniforms, outputs and first tile.
@@ -115,7 +118,7 @@ int launch_upload(const UploadSpan* spans, uint32_t num_spans, uint64_t total_ch
 // a readback fills (h_keys[i] = key of the gid in h_queue[i]).
 struct FeedbackExport {
   const unsigned long long* fb_slot;
-  const uint32_t* fb_count;
+  uint32_t* fb_state;  // [0] queue count, [1] stamp (set to next_stamp at the end)
   const uint32_t* fb_queue;
   const unsigned long long* fb_gid;
   uint32_t num_slots;
@@ -124,9 +127,12 @@ struct FeedbackExport {
   uint32_t* h_count;
   uint32_t* h_queue;
   unsigned long long* h_keys;
+  uint32_t next_stamp;
+  uint32_t pad;
 };
 
-// Enqueues the readback of one analysis cycle into the pinned buffers.
+// Enqueues the readback of one analysis cycle into the pinned buffers; the
+// kernel then starts the next cycle (device stamp = next_stamp).
 int launch_feedback_export(const FeedbackExport& x, void* stream);
 
 }  // namespace svo_impl

@@ -245,7 +245,8 @@ __device__ Hit traverse(const TracePools& S, const float o[3], float d[3], float
   // same state as the reference (pushes are credited per skipped period).
   uint64_t snap_path = 0;
   uint32_t snap_iter = 0, snap_scale = 0, snap_tmin = 0, snap_tmax = 0, snap_pushes = 0;
-  // outcome: 0 running, 1 hit (leaf / LOD stop), 2 miss, 3 maxLoops, 4 corrupt offset
+  // outcome: 0 running, 1 hit (leaf / LOD stop), 2 miss, 3 maxLoops, 4 corrupt
+  // offset or push below the stack (reported as a miss)
   uint32_t outcome = 0, hit_node = 0, hit_kind = 0, hit_ci = 0;
   float hit_tcmin = 0.0f;
 
@@ -312,7 +313,10 @@ __device__ Hit traverse(const TracePools& S, const float o[3], float d[3], float
           hit_kind = is_leaf ? 1u : 2u;
           hit_ci = ci;
           hit_tcmin = tc_min;
-        } else if (child >= S.pool_nodes) {  // corrupt offset: never read outside the pool
+        } else if (child >= S.pool_nodes || (POLICY == 1 && scale < mps)) {
+          // corrupt offset (never read outside the pool), or a push below
+          // the stack: policy 1 has no depth stop, so an inner node deeper
+          // than the stack holds (a malformed tree) ends the ray here
           outcome = 4;
         } else {  // push
           ++pushes;
@@ -524,8 +528,8 @@ __device__ Hit traverse_fast(const Uniform& U, const float o[3], const float d[3
             hit_kind = is_leaf ? 1u : 2u;
             hit_ci = ci;
             hit_err = is_leaf ? leaf_error(se, ratio, tc_min) : stop_error(se, ratio, tc_min);
-          } else if (child >= U.pool_nodes) {
-            outcome = 4;
+          } else if (child >= U.pool_nodes || (POLICY == 1 && scale < U.mps)) {
+            outcome = 4;  // corrupt offset, or (policy 1) a push below the stack
           } else {
             ++pushes;
             stk.put2(scale - U.mps, nidx, tmax);
@@ -609,15 +613,16 @@ __device__ __forceinline__ uint32_t float_key(float f) {
 // holds a sub-treelet gid asks for that treelet (largest error wins; the
 // first record of a cycle appends the gid to the queue).
 __device__ __forceinline__ void record_feedback(const TracePools& S, const Hit& hit) {
-  const unsigned long long key = ((unsigned long long)S.fb_stamp << 32) | float_key(hit.err);
+  const uint32_t stamp = S.fb_state[1];  // the current cycle (advanced by feedback_export_kernel)
+  const unsigned long long key = ((unsigned long long)stamp << 32) | float_key(hit.err);
   const uint32_t slot = (S.slot_shift < 32) ? (hit.node >> S.slot_shift) : (hit.node / S.nodes_per_treelet);
   atomicMax(&S.fb_slot[slot], key);
   if (hit.kind == 1u) {
     const uint32_t sub = (uint32_t)S.nodes[hit.node];  // a leaf's first child: its sub-treelet gid, 0 if none
     if (sub != 0u && sub < S.num_gids) {
       const unsigned long long old = atomicMax(&S.fb_gid[sub], key);
-      if ((uint32_t)(old >> 32) != S.fb_stamp) {
-        const uint32_t i = atomicAdd(S.fb_count, 1u);
+      if ((uint32_t)(old >> 32) != stamp) {
+        const uint32_t i = atomicAdd(&S.fb_state[0], 1u);
         if (i < S.fb_cap) S.fb_queue[i] = sub;
       }
     }
@@ -904,19 +909,25 @@ __global__ __launch_bounds__(256) void upload_kernel(const UploadSpan* __restric
 // whole, the count of asked gids as recorded, and the first min(count,
 // num_gids) queue entries with the key of each (the per-gid table stays on
 // the device).  Stores go straight over PCIe; the host reads the buffers
-// after a stream sync.
+// after a stream sync.  Lane 0 then advances the device stamp: launches
+// ordered after this one record into the next cycle.  (Only the count word
+// is read by the other lanes; the queue count itself is reset by a copy
+// behind this kernel.)
 __global__ __launch_bounds__(256) void feedback_export_kernel(FeedbackExport x) {
   const uint32_t stride = gridDim.x * blockDim.x;
   const uint32_t t0 = blockIdx.x * blockDim.x + threadIdx.x;
   for (uint32_t s = t0; s < x.num_slots; s += stride) x.h_slot[s] = x.fb_slot[s];
-  const uint32_t count = *x.fb_count;
+  const uint32_t count = x.fb_state[0];
   const uint32_t n = count < x.num_gids ? count : x.num_gids;
   for (uint32_t i = t0; i < n; i += stride) {
     const uint32_t g = x.fb_queue[i];
     x.h_queue[i] = g;
     x.h_keys[i] = g < x.num_gids ? x.fb_gid[g] : 0ull;
   }
-  if (t0 == 0) *x.h_count = count;
+  if (t0 == 0) {
+    *x.h_count = count;
+    x.fb_state[1] = x.next_stamp;
+  }
 }
 
 using TraceFn = void (*)(const TraceFrame*, uint32_t, uint32_t);

@@ -610,6 +610,11 @@ void sample_fill(const Node* nodes, uint64_t n, const float o[3], float d[3], fl
           std::memcpy(r->center, cc, sizeof cc);
           return;
         }
+        // A push from scale 0 would write stack[-1] (the reference's stack
+        // has levels 0..14 and no depth stop): only an inner node below
+        // depth 15, i.e. a malformed tree, gets here.  The ray ends as a
+        // miss, as for a child index outside the buffer.
+        if (scale == 0) { r->t = t_max; r->loops = while_counter; return; }
         parent->parent_tmin = tc_max;
         --scale;
         stack[scale] = StackElement{child, tc_min, tc_max, {cc[0], cc[1], cc[2]}};

@@ -50,3 +50,30 @@ def test_single_policy_needs_one_treelet(terrain8):
     mm = svo.TreeletMemoryManagerHip(sc, workloads.incore_bytes_for(sc), 0, resident_all=True)
     with pytest.raises(svo.SvoError):
         svo.render(mm, workloads.terrain_camera(1, 32, 32), "single")
+
+
+def test_push_below_the_stack_ends_the_ray(tmp_path):
+    """Policy B has no depth stop: in a tree with an inner node at depth 15
+    below every leaf (a chain of inner nodes towards the cube centre), a ray
+    through the chain pushes from scale 14 down to scale 1 and would push a
+    15th time from scale 0, below the 15-level stack.  Both the kernel and the
+    oracle end such a ray as a miss (t = the root box exit) after 14 pushes;
+    every other pixel matches too.  Regression test of the stack bound."""
+    import oracle
+    from tests.test_single_svo import _inner_chain, _write_fixture
+    nodes = _inner_chain(15)
+    packages = np.tile(np.array([[0x808080FF, 0x336699FF]], np.uint32), (nodes.shape[0], 1))
+    base = str(tmp_path / "chain15")
+    _write_fixture(base, "v1 15 %d " % nodes.shape[0], nodes, packages, aidx=list(range(nodes.shape[0])))
+    single = svo.Scene.open_single(base)
+    assert single.info["max_depth"] == 15
+    mm = svo.TreeletMemoryManagerHip(single, workloads.incore_bytes_for(single), 0, resident_all=True)
+    # the centre pixel's ray points at (1e-6,)*3, inside the depth-15 box [0, 2^-15]^3
+    cam = svo.look_at((2.0, 2.1, 1.9), (1e-6, 1e-6, 1e-6), (0.0, 1.0, 0.0), 2.0, 64, 64)
+    print("policy B, inner chain to depth 15", flush=True)
+    gpu = svo.render(mm, cam, "single")
+    ref = oracle.OracleSingle(base).render(cam)
+    _same(gpu, ref, "inner chain")
+    assert np.array_equal(gpu["pushes"], ref["pushes"])
+    c = gpu["pushes"][32, 32]
+    assert c == 14 and gpu["node"][32, 32] == svo.MISS, (c, gpu["node"][32, 32])

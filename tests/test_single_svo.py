@@ -217,3 +217,40 @@ def test_policy_b_agrees_with_policy_a_except_boundary(tmp_path, menger8):
     # or entry offset falls between them takes different steps without being
     # within 1e-6 of either policy's own threshold -- rare, not zero
     assert int((diff & ~flagged).sum()) <= cam.width * cam.height // 1000
+
+
+def _inner_chain(depth):
+    """Absolute nodes of a tree whose root has a leaf child 0 (depth 1) and an
+    inner child 7 that starts a chain of inner nodes, each the child 0 of the
+    one before, down to a childless inner node at `depth` (below every
+    leaf): the octants around the centre of the cube on the positive side."""
+    nodes = [[1, (0x81 << 8) | 0x01], [0, 0]]  # root: children 0 (leaf) and 7
+    for d in range(1, depth):  # inner node of depth d at index d+1, its child 0 next
+        nodes.append([d + 2, 0x01 << 8])
+    nodes.append([0, 0])  # the childless inner node at `depth`
+    return np.array(nodes, np.uint32)
+
+
+def test_inner_chain_below_every_leaf_counts_for_the_depth(tmp_path):
+    """The depth of a single-SVO tree counts every valid node: a chain of
+    inner nodes below the deepest leaf makes the tree deeper than its leaves,
+    so a header that names only the leaf depth is refused, and the policy-B
+    stack bound (depth 15) covers every node a traversal can push."""
+    nodes = _inner_chain(2)
+    packages = np.zeros((nodes.shape[0], 2), np.uint32)
+    base = str(tmp_path / "chain_leafdepth")
+    _write_fixture(base, "v1 1 %d " % nodes.shape[0], nodes, packages, aidx=list(range(nodes.shape[0])))
+    with pytest.raises(svo.SvoError) as e:
+        svo.Scene.open_single(base)
+    assert e.value.status == svo._capi.SVO_ERR_FORMAT
+    base = str(tmp_path / "chain_fulldepth")
+    _write_fixture(base, "v1 2 %d " % nodes.shape[0], nodes, packages, aidx=list(range(nodes.shape[0])))
+    sc = svo.Scene.open_single(base)
+    assert sc.info["max_depth"] == 2 and sc.info["tree_leaves"] == 1
+    # a chain to depth 16 loads with depth 16, which the single-SVO policy
+    # (15 levels) refuses when a render is staged
+    nodes = _inner_chain(16)
+    packages = np.zeros((nodes.shape[0], 2), np.uint32)
+    base = str(tmp_path / "chain16")
+    _write_fixture(base, "v1 16 %d " % nodes.shape[0], nodes, packages, aidx=list(range(nodes.shape[0])))
+    assert svo.Scene.open_single(base).info["max_depth"] == 16
