@@ -74,16 +74,34 @@ def profiled_traffic(shape):
     return None
 
 
-def cpu_baseline(scene, cam, threads, frames=5):
-    """Oracle restatement of CpuRaycasterSingleRay3 (the reference CPU caster)
-    timed on the host: OpenMP over rows on `threads` threads, median of
-    `frames` full frames after one warm-up, plus one single-thread frame."""
+def host_cpus():
+    """CPU threads the baseline may use: the process's affinity mask, capped
+    by OMP_NUM_THREADS when it is set (the CPU share granted to the process
+    that drives one GPU), with the machine's core count for the record."""
+    affinity = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS")
+    threads = affinity
+    if omp and omp.isdigit() and int(omp) > 0:
+        threads = min(threads, int(omp))
+    return {"nproc": os.cpu_count(), "affinity_cpus": affinity, "omp_num_threads": omp, "threads": max(1, threads)}
+
+
+def _oracle_scene(scene):
     import oracle
     with tempfile.TemporaryDirectory() as d:
         base = os.path.join(d, "scene")
         scene.write(base)
         info = scene.info
-        ora = oracle.OracleScene(base, info["treelet_bytes"] * info["num_treelets"])
+        return oracle.OracleScene(base, info["treelet_bytes"] * info["num_treelets"])
+
+
+def cpu_baseline(scene, cam, cpus, frames=5, ora=None):
+    """Oracle restatement of CpuRaycasterSingleRay3 (the reference CPU caster)
+    timed on the host: OpenMP over rows on every thread the process may use,
+    median of `frames` full frames after one warm-up, plus one single-thread
+    frame and one frame of the treelet renderer's policy."""
+    ora = ora or _oracle_scene(scene)
+    threads = cpus["threads"]
     rays = cam.width * cam.height
     ora.render(cam, "cpu3", threads=threads, extras=False)
     times = []
@@ -103,13 +121,48 @@ def cpu_baseline(scene, cam, threads, frames=5):
         "unit": "Mrays/s",
         "cores": threads,
         "kind": "port",
+        "nproc": cpus["nproc"],
+        "affinity_cpus": cpus["affinity_cpus"],
+        "single_thread_mrays_per_s": round(rays / single / 1e6, 3),
+        "treelet_policy_mrays_per_s": round(rays / pol_a / 1e6, 3),
         "sample": (f"oracle CpuRaycasterSingleRay3 policy, same scene and camera, {cam.width}x{cam.height} rays, "
-                   f"median of {frames} frames after 1 warm-up on {threads} threads; single thread "
-                   f"{rays / single / 1e6:.2f} Mrays/s; treelet policy {rays / pol_a / 1e6:.2f} Mrays/s"),
+                   f"median of {frames} frames after 1 warm-up on {threads} threads (affinity mask "
+                   f"{cpus['affinity_cpus']} CPUs, OMP_NUM_THREADS={cpus['omp_num_threads']}, nproc "
+                   f"{cpus['nproc']}); single thread {rays / single / 1e6:.2f} Mrays/s; treelet policy "
+                   f"{rays / pol_a / 1e6:.2f} Mrays/s on {threads} threads"),
     }
 
 
-def reference_case_section(stream, threads, cpu=True, launches=50):
+def cpu_parity(ora, cam, gpu, threads):
+    """The CPU leg's check of the headline frame: the oracle renders it under
+    both policies with its boundary-ray flags (a ray passing within 1e-6 of
+    a child face, edge or pop threshold, a direction component below the
+    clamp, or a maxLoops trip); the GPU frames of both policies are compared
+    with them pixel by pixel (node, treelet, rgba, t bits)."""
+    import numpy as np
+    ref = {p: ora.render(cam, p, threads=threads, extras=True) for p in ("treelet", "cpu3")}
+    flagged = ref["treelet"]["boundary"].astype(bool) | ref["cpu3"]["boundary"].astype(bool)
+
+    def differ(a, b):
+        d = np.zeros(a["node"].shape, bool)
+        for k in ("node", "treelet", "rgba"):
+            d |= a[k] != b[k]
+        return d | (a["t"].view(np.uint32) != b["t"].view(np.uint32))
+
+    a_a, c_c = differ(gpu["treelet"], ref["treelet"]), differ(gpu["cpu3"], ref["cpu3"])
+    a_c = differ(gpu["treelet"], ref["cpu3"])
+    node_a_c = gpu["treelet"]["node"] != ref["cpu3"]["node"]
+    return {
+        "pixels": int(flagged.size), "boundary_flagged": int(flagged.sum()),
+        "gpu_treelet_vs_oracle_treelet_differ": int(a_a.sum()),
+        "gpu_cpu3_vs_oracle_cpu3_differ": int(c_c.sum()),
+        "gpu_treelet_vs_oracle_cpu3_differ": int(a_c.sum()),
+        "gpu_treelet_vs_oracle_cpu3_node_differ": int(node_a_c.sum()),
+        "of_which_boundary_flagged": int((a_c & flagged).sum()),
+    }
+
+
+def reference_case_section(stream, cpus, cpu=True, launches=50):
     """The reference's own CPU-runnable case: a level-4 Menger sponge at depth
     8 in one treelet, 512x512 primary rays from a seeded point on a sphere of
     radius 2.5 (60 degree field of view), both GPU policies autotuned, next
@@ -131,10 +184,11 @@ def reference_case_section(stream, threads, cpu=True, launches=50):
         res[policy] = {"ms_per_launch": round(ms, 5), "mrays_per_s": round(rays / ms / 1e3, 1), "launch_shape": list(shape)}
         log(f"reference case, {policy} policy: {shape} {ms:.4f} ms/launch = {rays / ms / 1e3:.1f} Mrays/s")
     if cpu:
-        base = cpu_baseline(sc, cam, threads)
+        base = cpu_baseline(sc, cam, cpus)
         res["cpu_baseline"] = base
         res["gpu_over_cpu"] = round(res["cpu3"]["mrays_per_s"] / base["value"], 1)
-        log(f"reference case on the CPU: {base['value']:.2f} Mrays/s on {threads} threads; "
+        res["gpu_over_single_cpu_thread"] = round(res["cpu3"]["mrays_per_s"] / base["single_thread_mrays_per_s"], 1)
+        log(f"reference case on the CPU: {base['value']:.2f} Mrays/s on {cpus['threads']} threads; "
             f"GPU (same policy) / CPU = {res['gpu_over_cpu']}x")
     return res
 
@@ -489,10 +543,17 @@ def main():
             "traffic": profiled_traffic(shape),
         },
     }
+    # both policies' frames (untimed) for the CPU leg's pixel check
+    gpu_frames = {}
+    for pol in ("treelet", "cpu3"):
+        f = svo.Frame.alloc(cam.width, cam.height)
+        svo.render_to_buffer(mm, [cam], [f], pol, stream)
+        stream.synchronize()
+        gpu_frames[pol] = f.fetch()
+    cpus = host_cpus()
     extra_scenes = {"terrain11": (scene, mm)}
     if not args.no_reference_case:
-        out["reference_case"] = reference_case_section(stream, min(16, os.cpu_count() or 1),
-                                                       cpu=not args.no_cpu_baseline)
+        out["reference_case"] = reference_case_section(stream, cpus, cpu=not args.no_cpu_baseline)
     if not args.no_batched:
         out["batched"] = batched_section(extra_scenes, stream, args.policy)
     if not args.no_ooc:
@@ -501,11 +562,14 @@ def main():
         rates = [float(r) for r in args.serving_rates.split(",") if r]
         out["serving"] = serving_section(extra_scenes, rates, args.serving_seconds, args.policy)
     if not args.no_cpu_baseline:
-        threads = min(16, os.cpu_count() or 1)
-        log(f"cpu baseline on {threads} threads")
-        out["cpu_baseline"] = cpu_baseline(scene, cam, threads)
-        log(f"cpu baseline {out['cpu_baseline']['value']:.2f} Mrays/s; GPU/CPU = "
-            f"{value / out['cpu_baseline']['value']:.1f}x")
+        log(f"cpu baseline on {cpus['threads']} threads ({cpus})")
+        ora = _oracle_scene(scene)
+        out["cpu_baseline"] = cpu_baseline(scene, cam, cpus, ora=ora)
+        out["gpu_over_cpu"] = round(value / out["cpu_baseline"]["value"], 1)
+        out["gpu_over_single_cpu_thread"] = round(value / out["cpu_baseline"]["single_thread_mrays_per_s"], 1)
+        log(f"cpu baseline {out['cpu_baseline']['value']:.2f} Mrays/s; GPU/CPU = {out['gpu_over_cpu']}x")
+        out["boundary"] = cpu_parity(ora, cam, gpu_frames, cpus["threads"])
+        log(f"boundary rays and differing pixels of the headline frame: {out['boundary']}")
     print(json.dumps(out), flush=True)
 
 

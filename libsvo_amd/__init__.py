@@ -333,12 +333,8 @@ class TreeletMemoryManagerHip:
         return {k: getattr(st, k) for k, _ in _capi.svo_feedback_stats._fields_}
 
     def getSlots(self) -> List[int]:
-        out = []
-        g = ctypes.c_int32()
-        for s in range(self.info["num_slots"]):
-            check(lib.svo_mm_slot_treelet(self._h, s, ctypes.byref(g)), "svo_mm_slot_treelet")
-            out.append(g.value)
-        return out
+        """getSlots()[i]._treeletGid: the gid held by each slot, -1 empty."""
+        return [g for g, _, _ in self.getSlotInfos()]
 
     def getSlotGid(self, gid: int) -> int:
         s = ctypes.c_int32()
@@ -426,9 +422,10 @@ class TreeletMemoryManagerHip:
         """getSlots() with the whole SlotInfo: (treelet gid, visibility,
         error) per slot."""
         n = self.info["num_slots"]
-        buf = (_capi.svo_slot_info * n)()
-        check(lib.svo_mm_get_slots(self._h, 0, n, buf), "svo_mm_get_slots")
-        return [(b.treelet_gid, b.visibility, b.error) for b in buf]
+        buf = np.empty(n, dtype=[("g", np.int32), ("v", np.int32), ("e", np.float32), ("r", np.uint32)])
+        check(lib.svo_mm_get_slots(self._h, 0, n, buf.ctypes.data_as(ctypes.POINTER(_capi.svo_slot_info))),
+              "svo_mm_get_slots")
+        return list(zip(buf["g"].tolist(), buf["v"].tolist(), buf["e"].tolist()))
 
     def getDevicePools(self) -> Dict[str, int]:
         """Device pointers of the HBM pools (getClIncoreBufferGid /

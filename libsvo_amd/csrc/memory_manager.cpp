@@ -919,16 +919,28 @@ extern "C" svo_status svo_mm_init_device_buffer(svo_mm* mm, void* stream) {
   if (!mm) { set_error("svo_mm_init_device_buffer: null handle"); return SVO_ERR_ARG; }
   if (mm->host_only()) { set_error("svo_mm_init_device_buffer: handle has no device pools"); return SVO_ERR_STATE; }
   HIP_TRY(hipSetDevice(mm->device));
-  // every slot goes up again within its extents; the rest of the pools is
-  // zero on both sides already
+  // the whole buffer, as the reference's initClBuffer: the pools are
+  // cleared, then every slot goes up within its extent (beyond it the host
+  // buffer is zero)
+  const hipStream_t st = (hipStream_t)stream;
+  const size_t pn = (size_t)mm->num_slots * mm->N;
+  HIP_TRY(hipMemsetAsync(mm->d_nodes, 0, pn * 8, st));
+  HIP_TRY(hipMemsetAsync(mm->d_colors, 0, pn * 4, st));
+  HIP_TRY(hipMemsetAsync(mm->d_normals, 0, pn * 4, st));
   for (uint32_t s = 0; s < mm->num_slots; ++s) {
     mm->refilled[s] = 1u;
+    mm->dev_extent[s] = 0u;
+    mm->dev_nrm_extent[s] = 0u;
+    if (!mm->nrm_stale[s]) {
+      mm->nrm_stale[s] = 1u;
+      mm->nrm_slots.push_back(s);
+    }
     mm->mark(s);
   }
-  svo_upload_stats st;
-  svo_status r = mm->update_device((hipStream_t)stream, &st);
+  svo_upload_stats us;
+  svo_status r = mm->update_device(st, &us);
   if (r != SVO_OK) return r;
-  return mm->sync_normals((hipStream_t)stream);
+  return mm->sync_normals(st);
 }
 
 // ------------------------------------------------------------ render hooks

@@ -76,5 +76,42 @@ int main(int argc, char** argv) {
   if (!fo || std::fwrite(host.data(), 4, host.size(), fo) != host.size()) return 3;
   std::fclose(fo);
   std::printf("frame %u %u\n", kW, kH);
+
+  // The device pools (getClIncoreBuffer / getClAttributeIncoreBuffers) equal
+  // the host incore buffers; after one more frame's update they still do
+  // (normals included); initClBuffer restores pools that were overwritten.
+  auto pools_equal = [&mm]() {
+    const svo_device_pools p = mm.getDevicePools();
+    unsigned long long cnt = 0;
+    const svo_node* hn = mm.getIncoreBufferData(&cnt);
+    const uint32_t *hnrm = nullptr, *hcol = nullptr;
+    mm.getAttributeIncoreBuffer(&hnrm, &hcol);
+    std::vector<svo_node> dn(cnt);
+    std::vector<uint32_t> dnrm(cnt), dcol(cnt);
+    std::vector<int32_t> dslot(p.num_slots);
+    svo::check(svo_memcpy_d2h(dn.data(), p.nodes, cnt * 8), "svo_memcpy_d2h");
+    svo::check(svo_memcpy_d2h(dnrm.data(), p.normals, cnt * 4), "svo_memcpy_d2h");
+    svo::check(svo_memcpy_d2h(dcol.data(), p.colors, cnt * 4), "svo_memcpy_d2h");
+    svo::check(svo_memcpy_d2h(dslot.data(), p.slot_gid, p.num_slots * 4ull), "svo_memcpy_d2h");
+    bool same = cnt == p.pool_nodes && p.device == 0 && p.nodes_per_treelet == mm.getNumNodesPerTreelet();
+    for (unsigned long long i = 0; same && i < cnt; ++i)
+      same = dn[i].first_child == hn[i].first_child && dn[i].masks == hn[i].masks && dnrm[i] == hnrm[i] &&
+             dcol[i] == hcol[i];
+    const std::vector<int> slots_now = mm.getSlots();
+    for (unsigned s = 0; same && s < p.num_slots; ++s) same = dslot[s] == slots_now[s];
+    return same;
+  };
+  std::printf("pools_equal %d\n", pools_equal() ? 1 : 0);
+  cam = orbit_camera(frames);
+  analyse.performAnalysePass(cam, kDivide, 0.0f, 0.0f);
+  const svo_feedback_stats st2 = mm.updateClientSideIncoreBuffer(&analyse);
+  mm.updateDeviceMemory();
+  std::printf("pools_after_update %u %d\n", st2.inserted, pools_equal() ? 1 : 0);
+  const svo_device_pools p = mm.getDevicePools();
+  svo::check(svo_memset_device(p.nodes, 0xAB, p.pool_nodes * 8), "svo_memset_device");
+  svo::check(svo_memset_device(p.normals, 0xCD, p.pool_nodes * 4), "svo_memset_device");
+  std::printf("pools_clobbered %d\n", pools_equal() ? 1 : 0);
+  mm.initClBuffer();
+  std::printf("pools_after_init %d\n", pools_equal() ? 1 : 0);
   return 0;
 }

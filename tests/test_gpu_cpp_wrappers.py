@@ -62,11 +62,27 @@ def test_cpp_frame_loop_matches_python(terrain8, tmp_path):
         pages += st["inserted"]
     want.append("slots " + " ".join(str(g) for g in mm.getSlots()))
     want.append(f"frame {W} {H}")
-    assert lines == want
+    assert lines[:len(want)] == want
+    # the device pools taken through the C++ class equal the host buffers,
+    # stay equal through one more update, and initClBuffer restores them
+    mm.performAnalysePass(_camera(frames), DIVIDE, (0.0, 0.0), "treelet", s)
+    st = mm.updateClientSideIncoreBuffer(s)
+    assert lines[len(want):] == ["pools_equal 1", f"pools_after_update {st['inserted']} 1", "pools_clobbered 0",
+                                 "pools_after_init 1"]
+    mm.updateDeviceMemory(s)
     print(f"{pages} page-ins over {frames} frames", flush=True)
     assert pages > 0
     raw = np.fromfile(out, np.uint32).reshape(3, H, W)
-    ref = svo.render(mm, _camera(frames - 1), "treelet")
+    # the frame file holds the last frame of the loop, before the extra update
+    ref_mm = svo.TreeletMemoryManagerHip(sc, slots * sc.info["treelet_bytes"], device=0)
+    for f in range(frames):
+        cam = _camera(f)
+        jx = np.float32(0.25) * np.float32(f % 3) - np.float32(0.25)
+        jy = np.float32(0.5) - np.float32(0.25) * np.float32(f % 5)
+        ref_mm.performAnalysePass(cam, DIVIDE, (float(jx), float(jy)), "treelet", s)
+        ref_mm.updateClientSideIncoreBuffer(s)
+        ref_mm.updateDeviceMemory(s)
+    ref = svo.render(ref_mm, _camera(frames - 1), "treelet")
     assert np.array_equal(raw[0], ref["t"].view(np.uint32))
     assert np.array_equal(raw[1], ref["node"])
     assert np.array_equal(raw[2], ref["rgba"])

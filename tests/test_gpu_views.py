@@ -199,3 +199,86 @@ def test_view_errors(scenes, menger8):
         r.render([svo.View(scenes["menger"], zero, f)])
     with pytest.raises(svo.SvoError):
         r.render([svo.View(scenes["menger"], cam, f)], policy=7)
+
+
+def _terrain_path(n, w, h, lod, seed=500):
+    return [workloads.terrain_camera(seed + i, w, h, lod=lod, terrain_seed=workloads.TERRAIN_SEED) for i in range(n)]
+
+
+def test_batch_feedback_feeds_every_cycle(terrain8):
+    """A graph batch whose view records analysis feedback keeps feeding the
+    out-of-core update cycle after cycle (the cycle stamp is read on the
+    device at replay time): page-ins, slot tables and pixels equal those of
+    the same views rendered with svo_render_views on a second manager."""
+    sc, _ = terrain8
+    tb = sc.info["treelet_bytes"]
+    a = svo.TreeletMemoryManagerHip(sc, 48 * tb, device=0)
+    b = svo.TreeletMemoryManagerHip(sc, 48 * tb, device=0)
+    cams = _terrain_path(5, 256, 160, lod=0.25)
+    r = svo.Renderer(0)
+    s = svo.Stream()
+    fa, fb = svo.Frame.alloc(256, 160), svo.Frame.alloc(256, 160)
+    batch = svo.Batch(r, [svo.View(a, cams[0], fa, feedback_stride=3)], "treelet", s)
+    pages = 0
+    for c, cam in enumerate(cams):
+        batch.set_cameras([cam], s)
+        batch.launch(s)
+        r.render([svo.View(b, cam, fb, feedback_stride=3)], "treelet", s)
+        s.synchronize()
+        _compare(fa.fetch(), fb.fetch(), f"cycle {c}")
+        sa = a.updateClientSideIncoreBuffer(s)
+        sb = b.updateClientSideIncoreBuffer(s)
+        _say(f"batch feedback cycle {c}: {sa}")
+        assert sa == sb, f"cycle {c}"
+        assert sa["visible_slots"] > 0, f"cycle {c}: the batch recorded nothing"
+        assert a.getSlots() == b.getSlots(), f"cycle {c}"
+        pages += sa["inserted"]
+        a.updateDeviceMemory(s)
+        b.updateDeviceMemory(s)
+    assert pages > 0
+    # a replay with slots waiting for upload is refused
+    a.removeTreeletFromIncoreBuffer(next(g for g in a.getSlots() if g > 0))
+    with pytest.raises(svo.SvoError):
+        batch.launch(s)
+    a.updateDeviceMemory(s)
+    batch.launch(s)
+    s.synchronize()
+
+
+def test_batch_camera_update_switches_view_mode(terrain8):
+    """svo_batch_set_cameras checks view modes and, when a view switches to a
+    mode that shades with normals, sends the normals that out-of-core
+    updates left stale before the next replay."""
+    import oracle
+    sc, _ = terrain8
+    tb = sc.info["treelet_bytes"]
+    mm = svo.TreeletMemoryManagerHip(sc, 48 * tb, device=0)
+    s = svo.Stream()
+    cams = _terrain_path(4, 200, 150, lod=0.25, seed=620)
+    for cam in cams[:3]:  # refill slots: their normals are stale on the device
+        mm.performAnalysePass(cam, 4, (0.0, 0.0), "treelet", s)
+        mm.updateClientSideIncoreBuffer(s)
+        mm.updateDeviceMemory(s)
+    cam = cams[3]
+    f = svo.Frame.alloc(cam.width, cam.height, shade=True)
+    r = svo.Renderer(0)
+    batch = svo.Batch(r, [svo.View(mm, svo.with_view_mode(cam, "color"), f)], "treelet", s)
+    bad = svo.with_view_mode(cam, 99)
+    with pytest.raises(svo.SvoError):
+        batch.set_cameras([bad], s)
+    for mode in ("normal", "phong"):
+        vcam = svo.with_view_mode(cam, mode)
+        batch.set_cameras([vcam], s)
+        batch.launch(s)
+        s.synchronize()
+        g = f.fetch()
+        nodes, attr = mm.getIncoreBuffer()
+        ref = oracle.render_buffers(nodes, attr[:, 1], np.asarray(mm.getSlots(), np.int32), mm.getNumNodesPerTreelet(),
+                                    mm.scene.info["max_depth"], vcam, "treelet", normals=attr[:, 0])
+        _compare(g, ref, f"{mode} after camera update")
+        if mode == "normal":
+            same = (g["shade"].view(np.uint32) == ref["shade"].view(np.uint32)) | (np.isnan(g["shade"]) &
+                                                                                  np.isnan(ref["shade"]))
+            assert same.all(), f"{int((~same).sum())} shaded values differ"
+        else:
+            assert np.allclose(g["shade"], ref["shade"], rtol=0, atol=2e-6)
