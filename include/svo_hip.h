@@ -490,6 +490,22 @@ svo_status svo_render_views_timed(svo_renderer* r, const svo_view* views,
                                   uint32_t n, int policy, void* stream,
                                   uint32_t iters, float* mean_ms);
 
+/* Allocation-free variant of svo_render_views, for callers that cannot
+ * allocate device or pinned memory (tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  The view descriptors
+ * (which svo_render_views keeps in its renderer's ring, allocated on first
+ * use) go to a fixed device scratch array that each GPU holds for the life
+ * of the process: at most SVO_NOALLOC_MAX_VIEWS views per call (larger calls
+ * fail with SVO_ERR_ARG); config = NULL takes the default launch shape.  The
+ * residency handles were created beforehand (they own the pools); outputs
+ * are the caller's.  Every call on a GPU shares that GPU's scratch: calls
+ * ordered on one stream are safe, calls on different streams of one GPU that
+ * overlap race.  Results are those of svo_render_views.  Asynchronous on
+ * `stream` (the descriptors are copied from host memory before it returns). */
+enum { SVO_NOALLOC_MAX_VIEWS = 256 };
+svo_status svo_render_views_noalloc(const svo_view* views, uint32_t n, int policy,
+                                    const svo_launch_config* config, void* stream);
+
 /* Times every supported launch shape on these views (`iters` launches each,
  * after one warm-up) and installs the fastest in the renderer.  Results are
  * identical for every shape; only the speed differs. */

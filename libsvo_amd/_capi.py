@@ -27,6 +27,7 @@ SVO_POLICY_CPU3 = 2
 SVO_BUILD_PACK_BFS = 1
 SVO_SINGLE_SVOLOADER, SVO_SINGLE_V1 = 1, 2
 SVO_STACK_LDS, SVO_STACK_PRIVATE = 0, 1
+SVO_NOALLOC_MAX_VIEWS = 256
 VIEW_MODES = {"color": 0, "normal": 1, "iterations": 2, "svo_depth": 3, "t": 4, "loops": 5, "phong": 6, "quality": 7}
 MISS = 0xFFFFFFFF
 BACKGROUND_RGBA = 0x334D33FF
@@ -155,6 +156,7 @@ SIGNATURES = {
     "svo_renderer_set_config": (ctypes.c_int, [vp, P(svo_launch_config)]),
     "svo_renderer_get_config": (ctypes.c_int, [vp, P(svo_launch_config)]),
     "svo_render_views": (ctypes.c_int, [vp, P(svo_view), u32, ctypes.c_int, vp]),
+    "svo_render_views_noalloc": (ctypes.c_int, [P(svo_view), u32, ctypes.c_int, P(svo_launch_config), vp]),
     "svo_render_views_timed": (ctypes.c_int, [vp, P(svo_view), u32, ctypes.c_int, vp, u32, P(f32)]),
     "svo_autotune": (ctypes.c_int, [vp, P(svo_view), u32, ctypes.c_int, vp, u32, P(svo_launch_config), P(f32)]),
     "svo_batch_create": (ctypes.c_int, [vp, P(svo_view), u32, ctypes.c_int, vp, P(vp)]),

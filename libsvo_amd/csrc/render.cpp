@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 #include "svo_internal.h"
@@ -32,12 +33,10 @@ constexpr uint32_t kMaxViews = 1u << 16;
 
 namespace svo_impl {
 
-svo_status stage_views(const svo_view* views, uint32_t n, int policy, const LaunchShape& shape, int device,
-                       StagedLaunch* out) {
+svo_status stage_views_into(const svo_view* views, uint32_t n, int policy, const LaunchShape& shape, int device,
+                            TraceFrame* frames, StageTotals* totals) {
   if (!views || n == 0 || n > kMaxViews) { set_error("need 1..%u views", kMaxViews); return SVO_ERR_ARG; }
   const uint32_t tw = shape.tile_w, th = 64u / shape.tile_w;
-  out->frames.resize(n);
-  out->normal_handles.clear();
   uint64_t tiles = 0;
   uint32_t levels = 1;
   bool shade = false;
@@ -50,7 +49,7 @@ svo_status stage_views(const svo_view* views, uint32_t n, int policy, const Laun
       set_error("view %u: frame %ux%u out of range", i, c.width, c.height);
       return SVO_ERR_ARG;
     }
-    TraceFrame& F = out->frames[i];
+    TraceFrame& F = frames[i];
     svo_status r = mm_trace_pools(v.mm, policy, &F.pools);
     if (r != SVO_OK) return r;
     F.cam = c;
@@ -64,14 +63,31 @@ svo_status stage_views(const svo_view* views, uint32_t n, int policy, const Laun
     tiles += (uint64_t)F.tiles_x * ((c.height + th - 1) / th);
     levels = std::max(levels, F.pools.stack_levels);
     shade = shade || v.out.shade != nullptr;
-    if (v.out.shade && (c.view_mode == SVO_VIEW_NORMAL || c.view_mode == SVO_VIEW_PHONG) &&
-        std::find(out->normal_handles.begin(), out->normal_handles.end(), v.mm) == out->normal_handles.end())
-      out->normal_handles.push_back(v.mm);
   }
   if (tiles > 0x7FFFFFFFull) { set_error("launch of %llu tiles too large", (unsigned long long)tiles); return SVO_ERR_ARG; }
-  out->num_tiles = (uint32_t)tiles;
-  out->stack_levels = levels;
-  out->shade = shade;
+  *totals = StageTotals{(uint32_t)tiles, levels, shade};
+  return SVO_OK;
+}
+
+bool view_needs_normals(const svo_view& v) {
+  return v.out.shade && (v.camera.view_mode == SVO_VIEW_NORMAL || v.camera.view_mode == SVO_VIEW_PHONG);
+}
+
+svo_status stage_views(const svo_view* views, uint32_t n, int policy, const LaunchShape& shape, int device,
+                       StagedLaunch* out) {
+  if (!views || n == 0 || n > kMaxViews) { set_error("need 1..%u views", kMaxViews); return SVO_ERR_ARG; }
+  out->frames.resize(n);
+  out->normal_handles.clear();
+  StageTotals t{};
+  const svo_status r = stage_views_into(views, n, policy, shape, device, out->frames.data(), &t);
+  if (r != SVO_OK) return r;
+  for (uint32_t i = 0; i < n; ++i)
+    if (view_needs_normals(views[i]) &&
+        std::find(out->normal_handles.begin(), out->normal_handles.end(), views[i].mm) == out->normal_handles.end())
+      out->normal_handles.push_back(views[i].mm);
+  out->num_tiles = t.num_tiles;
+  out->stack_levels = t.stack_levels;
+  out->shade = t.shade;
   return SVO_OK;
 }
 
@@ -262,6 +278,48 @@ extern "C" svo_status svo_autotune(svo_renderer* r, const svo_view* views, uint3
   r->shape = bs;
   if (best) *best = svo_launch_config{bs.tile_w, bs.waves, bs.stack, 0};
   if (best_ms) *best_ms = bm;
+  return SVO_OK;
+}
+
+// ------------------------------------------------------------ allocation-free launches
+
+namespace {
+// Host image of the descriptors of one allocation-free launch (static
+// storage: staging allocates nothing), serialised by a lock.
+std::mutex g_noalloc_lock;
+TraceFrame g_noalloc_stage[kNoAllocMaxViews];
+}  // namespace
+
+extern "C" svo_status svo_render_views_noalloc(const svo_view* views, uint32_t n, int policy,
+                                               const svo_launch_config* config, void* stream) {
+  if (!views || n == 0) { set_error("svo_render_views_noalloc: no views"); return SVO_ERR_ARG; }
+  if (n > kNoAllocMaxViews) {
+    set_error("svo_render_views_noalloc: %u views, at most %u (its descriptor scratch holds %u)", n, kNoAllocMaxViews,
+              kNoAllocMaxViews);
+    return SVO_ERR_ARG;
+  }
+  if (!views[0].mm) { set_error("svo_render_views_noalloc: view 0 has no residency handle"); return SVO_ERR_ARG; }
+  LaunchShape sh{8, 1, SVO_STACK_LDS};
+  if (config) {
+    const svo_status r = shape_from_config(config, &sh);
+    if (r != SVO_OK) return r;
+  }
+  const int device = mm_device(views[0].mm);
+  if (device < 0) { set_error("svo_render_views_noalloc: handle has no device pools"); return SVO_ERR_STATE; }
+  HIP_TRY(hipSetDevice(device));
+  TraceFrame* d = nullptr;
+  int e = noalloc_scratch(device, &d);
+  if (e != 0) { set_error("svo_render_views_noalloc: scratch address: %s", hipGetErrorString((hipError_t)e)); return SVO_ERR_HIP; }
+  std::lock_guard<std::mutex> g(g_noalloc_lock);
+  StageTotals t{};
+  svo_status st = stage_views_into(views, n, policy, sh, device, g_noalloc_stage, &t);
+  if (st != SVO_OK) return st;
+  for (uint32_t i = 0; i < n; ++i)
+    if (view_needs_normals(views[i]) && (st = mm_sync_normals(views[i].mm, stream)) != SVO_OK) return st;
+  // pageable source: the copy has consumed the host image when it returns
+  HIP_TRY(hipMemcpyAsync(d, g_noalloc_stage, (size_t)n * sizeof(TraceFrame), hipMemcpyHostToDevice, (hipStream_t)stream));
+  e = launch_trace(sh, d, n, t.num_tiles, t.stack_levels, policy, t.shade, stream);
+  if (e != 0) { set_error("trace launch failed: %s", hipGetErrorString((hipError_t)e)); return SVO_ERR_HIP; }
   return SVO_OK;
 }
 

@@ -82,6 +82,26 @@ struct StagedLaunch {
   std::vector<svo_mm*> normal_handles;
 };
 
+// Tiles, stack depth and shading of a set of staged views.
+struct StageTotals {
+  uint32_t num_tiles;
+  uint32_t stack_levels;
+  bool shade;
+};
+// Writes the descriptors of views[0..n) into frames[0..n) (no allocation).
+svo_status stage_views_into(const svo_view* views, uint32_t n, int policy, const LaunchShape& shape, int device,
+                            TraceFrame* frames, StageTotals* totals);
+// The view shades with the normals pool (normal and Phong view modes).
+bool view_needs_normals(const svo_view& v);
+
+// Device scratch of the allocation-free launch: descriptors of at most
+// kNoAllocMaxViews views, in a fixed array that each GPU holds for the life
+// of the process.  noalloc_scratch returns this device's address of it (the
+// current device must be `device`), looked up once per device.  Returns a
+// hipError_t value.
+constexpr uint32_t kNoAllocMaxViews = SVO_NOALLOC_MAX_VIEWS;
+int noalloc_scratch(int device, TraceFrame** ptr);
+
 // Sends the stale normals of a staged launch's handles (see
 // mm_sync_normals) before it runs on `stream`.
 svo_status sync_staged_normals(const StagedLaunch& sg, void* stream);

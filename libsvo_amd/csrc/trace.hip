@@ -44,6 +44,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 
 #include "trace.h"
 
@@ -930,6 +931,12 @@ __global__ __launch_bounds__(256) void feedback_export_kernel(FeedbackExport x) 
   }
 }
 
+// View descriptors of the allocation-free launch (svo_render_views_noalloc):
+// a fixed array in device memory, one copy per GPU for the life of the
+// process, shared by every such launch on that GPU.  Host code never names
+// it: it uses the device address noalloc_scratch looks up.
+static __device__ __align__(256) TraceFrame scratch[kNoAllocMaxViews];
+
 using TraceFn = void (*)(const TraceFrame*, uint32_t, uint32_t);
 
 template <int POLICY, int TW, int WAVES>
@@ -957,6 +964,23 @@ TraceFn pick_tile(const LaunchShape& s) {
 }
 
 }  // namespace
+
+int noalloc_scratch(int device, TraceFrame** ptr) {
+  constexpr int kMaxDevices = 64;
+  static std::mutex lock;
+  static void* addr[kMaxDevices] = {};
+  if (device < 0 || device >= kMaxDevices) return (int)hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> g(lock);
+  if (!addr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return (int)e;
+    if (!p) return (int)hipErrorInvalidSymbol;
+    addr[device] = p;
+  }
+  *ptr = static_cast<TraceFrame*>(addr[device]);
+  return 0;
+}
 
 bool shape_supported(const LaunchShape& s) {
   return (s.tile_w == 8 || s.tile_w == 16 || s.tile_w == 32) && (s.waves == 1 || s.waves == 2 || s.waves == 4) &&
