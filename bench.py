@@ -74,6 +74,37 @@ def profiled_traffic(shape):
     return None
 
 
+VALU_CEILING_PER_CU_CYCLE = 2.0  # gfx950: one wave64 VALU instruction per SIMD every 2 cycles, 4 SIMDs per CU
+
+
+def profiled_issue(shape):
+    """VALU issue of the 1080p depth-11 frame for a launch shape, from the
+    issue-counter table of the committed profile: instructions per CU-cycle
+    against the 2.0 ceiling, SIMD busy fraction and lane use (None when the
+    shape has no row there)."""
+    import re
+    try:
+        text = open(PROFILE).read()
+    except OSError:
+        return None
+    section = text.find("## Issue counters per launch shape")
+    if section < 0:
+        return None
+    key = f"| ({shape[0]}, {shape[1]}, {shape[2]}) |"
+    for line in text[section:].splitlines():
+        if line.startswith(key):
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            try:
+                valu, busy, lane = float(cells[3]), float(cells[4]), float(cells[5])
+            except (IndexError, ValueError):
+                return None
+            return {"bound": "valu_issue", "achieved": valu, "peak": VALU_CEILING_PER_CU_CYCLE,
+                    "unit": "VALU instructions per CU-cycle", "frac": round(valu / VALU_CEILING_PER_CU_CYCLE, 3),
+                    "simd_busy": busy, "frac_while_busy": round(valu / busy / VALU_CEILING_PER_CU_CYCLE, 3),
+                    "lane_use": lane}
+    return None
+
+
 def host_cpus():
     """CPU threads the baseline may use: the process's affinity mask, capped
     by OMP_NUM_THREADS when it is set (the CPU share granted to the process
@@ -162,7 +193,7 @@ def cpu_parity(ora, cam, gpu, threads):
     }
 
 
-def reference_case_section(stream, cpus, cpu=True, launches=50):
+def reference_case_section(stream, cpus, cpu=True, launches=50, shape=None):
     """The reference's own CPU-runnable case: a level-4 Menger sponge at depth
     8 in one treelet, 512x512 primary rays from a seeded point on a sphere of
     radius 2.5 (60 degree field of view), both GPU policies autotuned, next
@@ -176,9 +207,14 @@ def reference_case_section(stream, cpus, cpu=True, launches=50):
     view = [svo.View(mm, cam, svo.Frame.alloc(cam.width, cam.height))]
     res = {"workload": "Menger sponge level 4, depth 8, one treelet, 512x512 primary rays, LOD off",
            "tree_nodes": sc.info["tree_nodes"]}
+    given = shape
     for policy in ("treelet", "cpu3"):
         r = svo.Renderer(0)
-        shape, _ = r.autotune(view, policy, stream, iters=10)
+        if given:
+            r.config = given
+            shape = r.config
+        else:
+            shape, _ = r.autotune(view, policy, stream, iters=10)
         r.timed(view, policy, stream, iters=5)
         ms = r.timed(view, policy, stream, iters=launches)
         res[policy] = {"ms_per_launch": round(ms, 5), "mrays_per_s": round(rays / ms / 1e3, 1), "launch_shape": list(shape)}
@@ -252,10 +288,69 @@ def batched_section(cache, stream, policy, launches=10):
     }
 
 
-def ooc_section(policy, frames, depth=13, treelet_bytes=131072, pool_bytes=8 << 30, divide=8):
-    """Out-of-core fly-through: analyse -> feedback update -> upload -> render
-    per frame, starting from the root treelet alone."""
+def _ooc_flight(mm, cams, policy, divide, renderer, stream, fused=False):
+    """One fly-through from the manager's current residency, one stage
+    after the other per frame: the analyse pass (or, fused, nothing: the
+    previous frame's render recorded feedback for every divide-th pixel per
+    axis), the feedback update (visibility, page-ins, evictions), the upload
+    and the render, each timed."""
     import numpy as np
+    import libsvo_amd as svo
+    from libsvo_amd import workloads
+    rng = np.random.default_rng(workloads.FLIGHT_SEED)
+    frame = svo.Frame.alloc(cams[0].width, cams[0].height)
+    view = [svo.View(mm, cams[0], frame, feedback_stride=divide if fused else 0)]
+    pages, evictions, failed, upload_bytes, moved_bytes = [], [], [], [], []
+    t_feedback = t_analyse = t_upload = t_render = 0.0
+    wall0 = time.perf_counter()
+    for i, cam in enumerate(cams):
+        a = time.perf_counter()
+        jit = tuple(float(v) for v in rng.uniform(-1.0, 1.0, 2))
+        if not fused:
+            mm.performAnalysePass(cam, divide, jit, policy, stream)
+            stream.synchronize()
+        a2 = time.perf_counter()
+        st = mm.updateClientSideIncoreBuffer(stream)
+        b = time.perf_counter()
+        up = mm.updateDeviceMemory(stream)
+        c = time.perf_counter()
+        view[0].camera = cam
+        renderer.render(view, policy, stream)
+        stream.synchronize()
+        d = time.perf_counter()
+        t_feedback += b - a
+        t_analyse += a2 - a
+        t_upload += c - b
+        t_render += d - c
+        pages.append(st["inserted"])
+        evictions.append(st["evicted"])
+        failed.append(st["insert_failed"])
+        upload_bytes.append(up["bytes"])
+        moved_bytes.append(up["bytes_moved"])
+        if i % 100 == 0:
+            log(f"ooc frame {i}{' (fused feedback)' if fused else ''}: {st['inserted']} page-ins, {st['evicted']} "
+                f"evictions, {mm.info['resident_treelets']} resident, {up['bytes'] / 2**20:.1f} MiB uploaded")
+    wall = time.perf_counter() - wall0
+    n = len(cams)
+    return {
+        "frames": n, "frames_per_s": round(n / wall, 1), "ms_per_frame": round(wall * 1e3 / n, 3),
+        "ms_feedback": round(t_feedback * 1e3 / n, 3), "ms_analyse_pass": round(t_analyse * 1e3 / n, 3),
+        "ms_upload": round(t_upload * 1e3 / n, 3), "ms_render": round(t_render * 1e3 / n, 3),
+        "page_ins_per_frame": round(float(np.mean(pages)), 2), "page_ins_max": int(max(pages)),
+        "evictions_per_frame": round(float(np.mean(evictions)), 2), "evictions_total": int(sum(evictions)),
+        "insert_failed_total": int(sum(failed)),
+        "upload_mib_per_frame": round(float(np.mean(upload_bytes)) / 2**20, 2),
+        "upload_mib_moved_per_frame": round(float(np.mean(moved_bytes)) / 2**20, 2),
+        "resident_treelets_end": mm.info["resident_treelets"],
+    }, pages
+
+
+def ooc_section(policy, frames, depth=13, treelet_bytes=131072, pool_bytes=8 << 30, small_pool_bytes=1 << 30,
+                divide=8):
+    """Out-of-core fly-through: analyse -> feedback update -> upload -> render
+    per frame, starting from the root treelet alone; then the same flight
+    pipelined, with feedback recorded by the render itself, and behind a
+    pool smaller than the flight's working set (evictions every frame)."""
     import libsvo_amd as svo
     from libsvo_amd import workloads
     t0 = time.perf_counter()
@@ -268,60 +363,16 @@ def ooc_section(policy, frames, depth=13, treelet_bytes=131072, pool_bytes=8 << 
         f"({info['num_treelets'] * treelet_bytes / 2**30:.1f} GiB of nodes), pool {slots} slots "
         f"({mm.info['device_bytes'] / 2**30:.2f} GiB in HBM)")
     cams = workloads.flythrough_cameras(frames, lod=1.0)
-    rng = np.random.default_rng(workloads.FLIGHT_SEED)
-    frame = svo.Frame.alloc(cams[0].width, cams[0].height)
     r = svo.Renderer(0)
     s = svo.Stream()
-    view = [svo.View(mm, cams[0], frame)]
-    pages, evictions, upload_bytes, moved_bytes, failed = [], [], [], [], 0
-    t_feedback = t_analyse = t_upload = t_render = 0.0
-    wall0 = time.perf_counter()
-    for i, cam in enumerate(cams):
-        a = time.perf_counter()
-        jit = tuple(float(v) for v in rng.uniform(-1.0, 1.0, 2))
-        mm.performAnalysePass(cam, divide, jit, policy, s)
-        s.synchronize()
-        a2 = time.perf_counter()
-        st = mm.updateClientSideIncoreBuffer(s)
-        b = time.perf_counter()
-        up = mm.updateDeviceMemory(s)
-        c = time.perf_counter()
-        view[0].camera = cam
-        r.render(view, policy, s)
-        s.synchronize()
-        d = time.perf_counter()
-        t_feedback += b - a
-        t_analyse += a2 - a
-        t_upload += c - b
-        t_render += d - c
-        pages.append(st["inserted"])
-        evictions.append(st["evicted"])
-        failed += st["insert_failed"]
-        upload_bytes.append(up["bytes"])
-        moved_bytes.append(up["bytes_moved"])
-        if i % 50 == 0:
-            log(f"ooc frame {i}: {st['inserted']} page-ins, {st['evicted']} evictions, "
-                f"{mm.info['resident_treelets']} resident, {up['bytes'] / 2**20:.1f} MiB uploaded")
-    wall = time.perf_counter() - wall0
-    res = {
-        "workload": f"terrain depth {depth}, {N}-node treelets ({info['num_treelets']} treelets, "
-                    f"{info['num_treelets'] * treelet_bytes / 2**30:.1f} GiB of nodes), {slots}-slot pool "
-                    f"({mm.info['device_bytes'] / 2**30:.2f} GiB HBM), {frames} frames "
-                    f"{cams[0].width}x{cams[0].height} along a seeded Catmull-Rom fly-through, LOD 1 pixel, "
-                    f"analyse pass 1/{divide} per axis, starting from the root treelet",
-        "frames": frames, "frames_per_s": round(frames / wall, 1),
-        "ms_per_frame": round(wall * 1e3 / frames, 3),
-        "ms_feedback": round(t_feedback * 1e3 / frames, 3),
-        "ms_analyse_pass": round(t_analyse * 1e3 / frames, 3),
-        "ms_upload": round(t_upload * 1e3 / frames, 3),
-        "ms_render": round(t_render * 1e3 / frames, 3),
-        "page_ins_per_frame": round(float(np.mean(pages)), 2), "page_ins_max": int(max(pages)),
-        "page_ins_after_frame_30": round(float(np.mean(pages[30:])), 2) if frames > 30 else None,
-        "evictions_total": int(sum(evictions)), "insert_failed_total": failed,
-        "upload_mib_per_frame": round(float(np.mean(upload_bytes)) / 2**20, 2),
-        "upload_mib_moved_per_frame": round(float(np.mean(moved_bytes)) / 2**20, 2),
-        "resident_treelets_end": mm.info["resident_treelets"], "seed": workloads.FLIGHT_SEED,
-    }
+    res, pages = _ooc_flight(mm, cams, policy, divide, r, s)
+    res["workload"] = (f"terrain depth {depth}, {N}-node treelets ({info['num_treelets']} treelets, "
+                       f"{info['num_treelets'] * treelet_bytes / 2**30:.1f} GiB of nodes), {slots}-slot pool "
+                       f"({mm.info['device_bytes'] / 2**30:.2f} GiB HBM), {frames} frames "
+                       f"{cams[0].width}x{cams[0].height} along a seeded Catmull-Rom fly-through, LOD 1 pixel, "
+                       f"analyse pass 1/{divide} per axis, starting from the root treelet")
+    res["page_ins_after_frame_30"] = round(sum(pages[30:]) / len(pages[30:]), 2) if frames > 30 else None
+    res["seed"] = workloads.FLIGHT_SEED
     log(f"ooc: {res['frames_per_s']} frames/s, {res['page_ins_per_frame']} page-ins/frame, "
         f"{res['upload_mib_per_frame']} MiB/frame in merged ranges ({res['upload_mib_moved_per_frame']} MiB moved), "
         f"render {res['ms_render']} ms, "
@@ -333,6 +384,25 @@ def ooc_section(policy, frames, depth=13, treelet_bytes=131072, pool_bytes=8 << 
     mm.updateDeviceMemory(s)
     res["pipelined"] = _ooc_pipelined(mm, cams, divide, policy, pages, resident_end)
     log(f"ooc pipelined: {res['pipelined']}")
+    # feedback recorded by each frame's render (every divide-th pixel per
+    # axis, no jitter) drives the next frame's update: no analyse launch
+    mm.resetIncoreBuffer()
+    mm.updateDeviceMemory(s)
+    fused, _ = _ooc_flight(mm, cams, policy, divide, r, s, fused=True)
+    fused["note"] = (f"feedback recorded during the render of frame i (every {divide}th pixel per axis, no jitter) "
+                     f"drives the update before frame i+1; the reference's separate jittered 1/{divide} analyse "
+                     f"pass is not run, so residency lags one frame and differs from the line above")
+    res["fused_feedback"] = fused
+    log(f"ooc fused feedback: {fused}")
+    del mm
+    # a pool smaller than the flight's working set: freeIncorePosition evicts
+    small = small_pool_bytes // (N * 16)
+    mm2 = svo.TreeletMemoryManagerHip(sc, small * info["treelet_bytes"], device=0)
+    ev, _ = _ooc_flight(mm2, cams, policy, divide, r, s)
+    ev["workload"] = (f"the same flight behind a {small}-slot pool ({mm2.info['device_bytes'] / 2**30:.2f} GiB HBM), "
+                      f"smaller than the flight's working set")
+    res["evicting_pool"] = ev
+    log(f"ooc evicting pool: {ev}")
     return res
 
 
@@ -393,30 +463,46 @@ def serving_section(cache, rates, seconds, policy):
             srv.add_scene(_scene(cache, k)[1])
         servers[name] = srv
     scene_bytes = sum(_scene(cache, k)[1].info["device_bytes"] for k in keys)
+    def row_of(st):
+        return {"frames_per_s": round(st["frames_per_s"], 1), "mrays_per_s": round(st["mrays_per_s"], 1),
+                "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3), "max_ms": round(st["max_ms"], 3),
+                "mean_batch_views": round(st["mean_batch_views"], 2), "max_batch_views": st["max_batch_views_seen"],
+                "max_inflight": st["max_inflight_seen"]}
+
     results = []
     for rate in rates:
         reqs = workloads.serving_requests(kinds, rate, max(50, int(rate * seconds)))
-        row = {"rate_req_s": rate, "requests": len(reqs)}
+        span = reqs[-1][2] - reqs[0][2]
+        # the rate this sample of the Poisson process offers (requests over
+        # the span of their arrivals)
+        row = {"rate_req_s": rate, "requests": len(reqs),
+               "offered_req_s": round(len(reqs) / span, 1) if span > 0 else None}
         for name, srv in servers.items():
             st, comp = srv.replay(reqs)
-            row[name] = {"frames_per_s": round(st["frames_per_s"], 1), "mrays_per_s": round(st["mrays_per_s"], 1),
-                         "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3),
-                         "max_ms": round(st["max_ms"], 3), "mean_batch_views": round(st["mean_batch_views"], 2),
-                         "max_inflight": st["max_inflight_seen"]}
+            row[name] = row_of(st)
             log(f"serving {rate:.0f} req/s {name}: {row[name]}")
         results.append(row)
-    # saturation: every request queued at t=0
+    # saturation: every request queued at t=0 (latency is then mostly queueing)
     reqs = workloads.serving_requests(kinds, 1e9, 2000)
     sat = {}
     for name, srv in servers.items():
         st, _ = srv.replay(reqs)
-        sat[name] = {"frames_per_s": round(st["frames_per_s"], 1), "mrays_per_s": round(st["mrays_per_s"], 1)}
+        sat[name] = row_of(st)
         log(f"serving saturation {name}: {sat[name]}")
+    # the highest rate of the sweep each loop keeps up with (throughput
+    # within 5 % of the offered rate) at p99 <= 20 ms
+    sustained = {}
+    for name in servers:
+        ok = [row["rate_req_s"] for row in results
+              if row[name]["p99_ms"] <= 20.0 and row["offered_req_s"]
+              and row[name]["frames_per_s"] >= 0.95 * row["offered_req_s"]]
+        sustained[name] = max(ok) if ok else None
     return {
         "workload": "Menger d9 + terrain d11 + sphere cloud d10 + terrain d12 resident, Poisson arrivals, "
                     "resolutions 256^2 40% / 720p 40% / 1080p 15% / 4K 5%, random cameras; latency = "
                     "completion on device - arrival",
         "hbm_budget_bytes": budget, "scene_bytes": scene_bytes, "rates": results, "saturation": sat,
+        "max_rate_req_s_at_p99_le_20ms": sustained,
         "seed": workloads.SERVING_SEED, "seconds_per_rate": seconds,
     }
 
@@ -438,8 +524,15 @@ def main():
     ap.add_argument("--no-reference-case", action="store_true",
                     help="skip the reference's own case (Menger depth 8 in one treelet, 512x512)")
     ap.add_argument("--ooc-frames", type=int, default=300)
-    ap.add_argument("--serving-seconds", type=float, default=3.0, help="length of each request stream")
-    ap.add_argument("--serving-rates", default="200,1000,2000", help="Poisson arrival rates (req/s)")
+    ap.add_argument("--serving-seconds", type=float, default=2.0, help="length of each request stream")
+    ap.add_argument("--serving-rates", default="200,1000,2000,4000,6000,8000,10000,12000,14000",
+                    help="Poisson arrival rates (req/s)")
+    ap.add_argument("--launch-shape", default="",
+                    help="tile_width,waves,stack (e.g. 8,1,lds): use this shape for the headline instead of the "
+                         "autotuner's pick (profiling runs)")
+    ap.add_argument("--valu-calibration-blocks", type=int, default=0,
+                    help="after the timed launches, run the VALU issue probe with this many 256-lane "
+                         "workgroups (the VALU-busy reading of a kernel that only issues VALU instructions)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="after the timed launches, run one plain device copy of this many MiB "
                          "(known traffic for scaling FETCH_SIZE / WRITE_SIZE counters)")
@@ -478,8 +571,14 @@ def main():
     frame = svo.Frame.alloc(cam.width, cam.height)
     view = [svo.View(mm, cam, frame)]
     renderer = svo.Renderer(0)
-    shape, tune_ms = renderer.autotune(view, args.policy, stream, iters=20)  # shapes lie within a few %
-    log(f"autotuned launch shape {shape} ({tune_ms:.4f} ms in tuning)")
+    if args.launch_shape:
+        t, w, k = args.launch_shape.split(",")
+        renderer.config = (int(t), int(w), k)
+        shape = renderer.config
+        log(f"launch shape {shape} (given)")
+    else:
+        shape, tune_ms = renderer.autotune(view, args.policy, stream, iters=20)  # shapes lie within a few %
+        log(f"autotuned launch shape {shape} ({tune_ms:.4f} ms in tuning)")
     if args.warmup > 0:
         renderer.timed(view, args.policy, stream, iters=args.warmup)
     svo._capi.check(svo.lib.svo_device_synchronize(), "sync")
@@ -494,13 +593,22 @@ def main():
 
     # the other traversal policy on the same frame (autotuned on its own)
     other = "cpu3" if args.policy == "treelet" else "treelet"
-    oshape, _ = renderer.autotune(view, other, stream, iters=20)
+    if args.launch_shape:
+        oshape = shape
+    else:
+        oshape, _ = renderer.autotune(view, other, stream, iters=20)
     ms_other = renderer.timed(view, other, stream, iters=max(10, args.steps // 2))
     log(f"{other} policy: {oshape} {ms_other:.4f} ms/launch = {rays / ms_other / 1e3:.1f} Mrays/s")
     policies = {args.policy: {"mrays_per_s": round(value, 1), "ms_per_launch": round(ms, 5), "launch_shape": list(shape)},
                 other: {"mrays_per_s": round(rays / ms_other / 1e3, 1), "ms_per_launch": round(ms_other, 5),
                         "launch_shape": list(oshape)}}
 
+    if args.valu_calibration_blocks > 0:
+        blocks = args.valu_calibration_blocks
+        buf = svo.DeviceBuffer(blocks * 256 * 4)
+        svo._capi.check(svo.lib.svo_debug_valu(buf.ptr, blocks, 4096, stream.handle), "svo_debug_valu")
+        stream.synchronize()
+        log(f"VALU probe of {blocks} workgroups x 4096 iterations done")
     if args.copy_calibration_mb > 0:
         nbytes = args.copy_calibration_mb << 20
         src, dst = svo.DeviceBuffer(nbytes), svo.DeviceBuffer(nbytes)
@@ -541,6 +649,7 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5),
             "traffic": profiled_traffic(shape),
+            "issue": profiled_issue(shape),
         },
     }
     # both policies' frames (untimed) for the CPU leg's pixel check
@@ -553,7 +662,8 @@ def main():
     cpus = host_cpus()
     extra_scenes = {"terrain11": (scene, mm)}
     if not args.no_reference_case:
-        out["reference_case"] = reference_case_section(stream, cpus, cpu=not args.no_cpu_baseline)
+        out["reference_case"] = reference_case_section(stream, cpus, cpu=not args.no_cpu_baseline,
+                                                       shape=shape if args.launch_shape else None)
     if not args.no_batched:
         out["batched"] = batched_section(extra_scenes, stream, args.policy)
     if not args.no_ooc:

@@ -460,9 +460,14 @@ typedef struct svo_view {
  *   tile_width      : pixels per wavefront tile row, 8, 16 or 32 (the tile
  *                     is tile_width x 64/tile_width)
  *   waves_per_group : wavefronts per workgroup, 1, 2 or 4
- *   stack_layout    : SVO_STACK_LDS (shared memory, [level][lane]) or
- *                     SVO_STACK_PRIVATE (per-lane private array) */
-enum { SVO_STACK_LDS = 0, SVO_STACK_PRIVATE = 1 };
+ *   stack_layout    : SVO_STACK_LDS (shared memory, [level][lane], the
+ *                     parent's node index and tmax: a pop re-reads the
+ *                     parent node), SVO_STACK_PRIVATE (per-lane private
+ *                     array) or SVO_STACK_LDS_NODE (shared memory holding the
+ *                     parent node's child base and masks as well: a pop
+ *                     reads no global memory; 10 instead of 8 bytes per
+ *                     level and lane) */
+enum { SVO_STACK_LDS = 0, SVO_STACK_PRIVATE = 1, SVO_STACK_LDS_NODE = 2 };
 typedef struct svo_launch_config {
   uint32_t tile_width;
   uint32_t waves_per_group;
@@ -748,6 +753,14 @@ svo_status svo_debug_ray_dirs(const svo_camera* cam, float* dirs_dev,
  * plain streaming kernel; a dispatch of known traffic for calibrating the
  * FETCH_SIZE / WRITE_SIZE hardware counters. */
 svo_status svo_debug_copy(void* dst, const void* src, size_t bytes,
+                          void* stream);
+
+/* Diagnostic: a kernel that only issues VALU instructions (eight
+ * independent integer chains per lane, 16 VALU instructions per iteration,
+ * iterations rounded up to a multiple of 8, one store per lane), `blocks`
+ * workgroups of 256 lanes; `out` holds blocks * 256 words.  Its hardware counters give the VALU-busy reading of a
+ * kernel bound by VALU issue alone, the reference for the traversal's. */
+svo_status svo_debug_valu(uint32_t* out, uint32_t blocks, uint32_t iters,
                           void* stream);
 
 #ifdef __cplusplus

@@ -541,9 +541,10 @@ def ray_dirs(cam: svo_camera) -> np.ndarray:
 # ------------------------------------------------------------ multi-view launches
 
 LaunchConfig = svo_launch_config
-STACKS = {"lds": SVO_STACK_LDS, "private": SVO_STACK_PRIVATE}
+STACKS = {"lds": SVO_STACK_LDS, "private": SVO_STACK_PRIVATE, "lds_node": _capi.SVO_STACK_LDS_NODE}
+STACK_NAMES = {v: k for k, v in STACKS.items()}
 #: every (tile_width, waves_per_group, stack_layout) the kernels are built for
-LAUNCH_SHAPES = [(t, w, s) for t in (8, 16, 32) for w in (1, 2, 4) for s in ("lds", "private")]
+LAUNCH_SHAPES = [(t, w, s) for t in (8, 16, 32) for w in (1, 2, 4) for s in ("lds", "private", "lds_node")]
 
 
 @dataclass
@@ -588,7 +589,7 @@ class Renderer:
     def config(self):
         c = svo_launch_config()
         check(lib.svo_renderer_get_config(self._h, ctypes.byref(c)), "svo_renderer_get_config")
-        return (c.tile_width, c.waves_per_group, "lds" if c.stack_layout == SVO_STACK_LDS else "private")
+        return (c.tile_width, c.waves_per_group, STACK_NAMES[c.stack_layout])
 
     @config.setter
     def config(self, shape) -> None:

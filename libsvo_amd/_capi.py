@@ -26,7 +26,7 @@ SVO_REQ_DONE, SVO_REQ_EXPIRED, SVO_REQ_REJECTED = 0, 1, 2
 SVO_POLICY_CPU3 = 2
 SVO_BUILD_PACK_BFS = 1
 SVO_SINGLE_SVOLOADER, SVO_SINGLE_V1 = 1, 2
-SVO_STACK_LDS, SVO_STACK_PRIVATE = 0, 1
+SVO_STACK_LDS, SVO_STACK_PRIVATE, SVO_STACK_LDS_NODE = 0, 1, 2
 SVO_NOALLOC_MAX_VIEWS = 256
 VIEW_MODES = {"color": 0, "normal": 1, "iterations": 2, "svo_depth": 3, "t": 4, "loops": 5, "phong": 6, "quality": 7}
 MISS = 0xFFFFFFFF
@@ -190,6 +190,7 @@ SIGNATURES = {
     "svo_render_timed": (ctypes.c_int, [vp, P(svo_camera), u32, P(svo_frame_out), ctypes.c_int, vp, u32, P(f32)]),
     "svo_debug_ray_dirs": (ctypes.c_int, [P(svo_camera), vp, vp]),
     "svo_debug_copy": (ctypes.c_int, [vp, vp, ctypes.c_size_t, vp]),
+    "svo_debug_valu": (ctypes.c_int, [vp, u32, u32, vp]),
 }
 
 

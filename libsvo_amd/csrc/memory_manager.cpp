@@ -1266,6 +1266,13 @@ extern "C" svo_status svo_debug_ray_dirs(const svo_camera* cam, float* dirs_dev,
   return SVO_OK;
 }
 
+extern "C" svo_status svo_debug_valu(uint32_t* out, uint32_t blocks, uint32_t iters, void* stream) {
+  if (!out || blocks == 0 || blocks > (1u << 20)) { set_error("svo_debug_valu: null output or blocks not in 1..2^20"); return SVO_ERR_ARG; }
+  int e = launch_valu_probe(out, blocks, iters, stream);
+  if (e != 0) { set_error("VALU probe launch failed: %s", hipGetErrorString((hipError_t)e)); return SVO_ERR_HIP; }
+  return SVO_OK;
+}
+
 extern "C" svo_status svo_debug_copy(void* dst, const void* src, size_t bytes, void* stream) {
   if (!dst || !src || (bytes % 16) != 0) { set_error("svo_debug_copy: null pointer or size not a multiple of 16"); return SVO_ERR_ARG; }
   int e = launch_copy(dst, src, bytes, stream);

@@ -262,7 +262,8 @@ extern "C" svo_status svo_autotune(svo_renderer* r, const svo_view* views, uint3
                                    uint32_t iters, svo_launch_config* best, float* best_ms) {
   if (!r || iters == 0) { set_error("svo_autotune: bad arguments"); return SVO_ERR_ARG; }
   HIP_TRY(hipSetDevice(r->device));
-  const uint32_t tiles[] = {8, 16, 32}, waves[] = {1, 2, 4}, stacks[] = {SVO_STACK_LDS, SVO_STACK_PRIVATE};
+  const uint32_t tiles[] = {8, 16, 32}, waves[] = {1, 2, 4},
+                 stacks[] = {SVO_STACK_LDS, SVO_STACK_PRIVATE, SVO_STACK_LDS_NODE};
   float bm = 0;
   LaunchShape bs{};
   bool found = false;

@@ -59,7 +59,7 @@ struct TraceFrame {
 struct LaunchShape {
   uint32_t tile_w;  // 8, 16 or 32
   uint32_t waves;   // 1, 2 or 4
-  uint32_t stack;   // SVO_STACK_LDS or SVO_STACK_PRIVATE
+  uint32_t stack;   // SVO_STACK_LDS, SVO_STACK_PRIVATE or SVO_STACK_LDS_NODE
 };
 
 bool shape_supported(const LaunchShape& s);
@@ -116,6 +116,10 @@ int launch_ray_dirs(const svo_camera& cam, float* dirs_dev, void* stream);
 
 // Copies bytes/16 16-byte words with a plain streaming kernel.
 int launch_copy(void* dst, const void* src, size_t bytes, void* stream);
+
+// VALU issue probe: `blocks` workgroups of 256 lanes, `iters` iterations of
+// 16 VALU instructions each; out holds blocks * 256 words.
+int launch_valu_probe(uint32_t* out, uint32_t blocks, uint32_t iters, void* stream);
 
 // One span of an upload: `bytes` (a multiple of 4) from `src` (pinned host
 // memory or device memory) to `dst` (device memory); first_chunk is the

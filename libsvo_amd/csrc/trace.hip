@@ -174,6 +174,7 @@ template <int STACK, int BLOCK>
 struct RayStack;
 template <int BLOCK>
 struct RayStack<SVO_STACK_LDS, BLOCK> {  // LDS, [level][lane] of 8-byte entries
+  static constexpr bool kHoldsNodes = false;
   uint2* base;
   uint32_t tid;
   __device__ __forceinline__ void put(int l, uint32_t v) { base[l * BLOCK + tid].x = v; }
@@ -185,8 +186,34 @@ struct RayStack<SVO_STACK_LDS, BLOCK> {  // LDS, [level][lane] of 8-byte entries
     t = __uint_as_float(e.y);
   }
 };
+// LDS, [level][lane]: 8-byte entries {node word, tmax} and, in a second
+// array behind the first, the 16-bit child masks of the node left by a push.
+// The fast loop keeps the parent's node itself here (its child base, i.e.
+// node index + first-child offset, and its masks), so that a pop reads LDS
+// instead of re-reading the parent from global memory; the generic loop
+// keeps the node index in the first word, as with SVO_STACK_LDS.
+template <int BLOCK>
+struct RayStack<SVO_STACK_LDS_NODE, BLOCK> {
+  static constexpr bool kHoldsNodes = true;
+  uint2* base;
+  uint16_t* masks;
+  uint32_t tid;
+  __device__ __forceinline__ void put(int l, uint32_t v) { base[l * BLOCK + tid].x = v; }
+  __device__ __forceinline__ uint32_t get(int l) const { return base[l * BLOCK + tid].x; }
+  __device__ __forceinline__ void put3(int l, uint32_t cb, uint32_t m, float t) {
+    base[l * BLOCK + tid] = make_uint2(cb, __float_as_uint(t));
+    masks[l * BLOCK + tid] = (uint16_t)m;
+  }
+  __device__ __forceinline__ void get3(int l, uint32_t& cb, uint32_t& m, float& t) const {
+    const uint2 e = base[l * BLOCK + tid];
+    cb = e.x;
+    t = __uint_as_float(e.y);
+    m = masks[l * BLOCK + tid];
+  }
+};
 template <int BLOCK>
 struct RayStack<SVO_STACK_PRIVATE, BLOCK> {  // private (scratch-backed) arrays
+  static constexpr bool kHoldsNodes = false;
   uint32_t v[kMaxScale];
   float tm[kMaxScale];
   __device__ __forceinline__ void put(int l, uint32_t x) { v[l] = x; }
@@ -477,7 +504,9 @@ __device__ Hit traverse_fast(const Uniform& U, const float o[3], const float d[3
         ++scale;
         float tparent;
         uint32_t fidx;
-        stk.get2(scale - U.mps, fidx, tparent);
+        uint32_t pmask = 0;
+        if constexpr (Stack::kHoldsNodes) stk.get3(scale - U.mps, fidx, pmask, tparent);
+        else stk.get2(scale - U.mps, fidx, tparent);
         const uint32_t pl = (uint32_t)path;
         path >>= 3;
         const uint32_t seb = __float_as_uint(se);
@@ -488,7 +517,14 @@ __device__ Hit traverse_fast(const Uniform& U, const float o[3], const float d[3
         tmin = tmax;
         tmax = tparent;
         nidx = fidx;
-        nd = U.nodes[fidx];
+        if constexpr (Stack::kHoldsNodes) {
+          // the parent's child base and masks come back from the stack: no
+          // global load on a pop (nidx now holds the child base, nd the
+          // masks over a zero offset)
+          nd = (uint64_t)pmask << 32;
+        } else {
+          nd = U.nodes[fidx];
+        }
       }
     }
     // second half: the next reference iteration if it steps into a child
@@ -533,7 +569,8 @@ __device__ Hit traverse_fast(const Uniform& U, const float o[3], const float d[3
             outcome = 4;  // corrupt offset, or (policy 1) a push below the stack
           } else {
             ++pushes;
-            stk.put2(scale - U.mps, nidx, tmax);
+            if constexpr (Stack::kHoldsNodes) stk.put3(scale - U.mps, nidx + (uint32_t)nd, masks, tmax);
+            else stk.put2(scale - U.mps, nidx, tmax);
             path = (path << 3) | ci;
             --scale;
             const float h = se * 0.5f;
@@ -566,6 +603,17 @@ __device__ Hit traverse_fast(const Uniform& U, const float o[3], const float d[3
     r.quality = hit_kind == 1u ? hit_err : 0.0f;
   } else if (outcome == 3) {
     if (POLICY == 0) {  // maxLoops returns the current parent as a hit
+      if constexpr (Stack::kHoldsNodes) {
+        // after a pop nidx holds the node's child base, not its index: walk
+        // the path from the root to find the index (maxLoops rays only)
+        uint32_t n = 0;
+        for (int l = U.smax - 2 - scale; l >= 0; --l) {
+          const uint64_t v = U.nodes[n];
+          const uint32_t ci = (uint32_t)(path >> (3 * l)) & 7u;
+          n = n + (uint32_t)v + (uint32_t)__popc(((uint32_t)(v >> 32) >> 8) & ((1u << ci) - 1u));
+        }
+        nidx = n;
+      }
       r.node = nidx;
       r.t = tmin;
       r.depth = (uint32_t)(U.smax - scale);
@@ -742,7 +790,7 @@ __device__ float4 shade_pixel(uint32_t mode, const Hit& hit, const float d[3], c
 
 template <int POLICY, int TW, int WAVES, int STACK>
 __global__ __launch_bounds__(64 * WAVES) void trace_kernel(const TraceFrame* __restrict__ frames, uint32_t num_frames,
-                                                           uint32_t num_tiles) {
+                                                           uint32_t num_tiles, uint32_t stack_levels) {
   constexpr int kBlock = 64 * WAVES;
   constexpr uint32_t TH = 64u / TW;
   extern __shared__ uint32_t s_stack[];
@@ -783,6 +831,10 @@ __global__ __launch_bounds__(64 * WAVES) void trace_kernel(const TraceFrame* __r
   RayStack<STACK, kBlock> stk;
   if constexpr (STACK == SVO_STACK_LDS) {
     stk.base = reinterpret_cast<uint2*>(s_stack);
+    stk.tid = tid;
+  } else if constexpr (STACK == SVO_STACK_LDS_NODE) {
+    stk.base = reinterpret_cast<uint2*>(s_stack);
+    stk.masks = reinterpret_cast<uint16_t*>(s_stack + 2u * stack_levels * kBlock);
     stk.tid = tid;
   }
   float dt[3] = {d[0], d[1], d[2]};  // the traversal may clamp its copy
@@ -879,6 +931,34 @@ __global__ __launch_bounds__(256) void copy_kernel(uint4* __restrict__ dst, cons
     dst[i] = src[i];
 }
 
+// VALU issue probe: eight independent integer chains per lane (no memory
+// traffic but one store per lane at the end), 16 VALU instructions per
+// iteration, eight iterations per loop trip so that the loop's own scalar
+// instructions are few.  Launched with enough waves to fill every SIMD, its
+// counters give the VALU-busy reading of a kernel that does nothing but
+// issue VALU instructions, against which the traversal's reading is
+// compared.
+__global__ __launch_bounds__(256) void valu_probe_kernel(uint32_t* __restrict__ out, uint32_t iters) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t a[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) a[k] = g * (uint32_t)(2 * k + 1);
+  for (uint32_t i = 0; i < iters; i += 8u) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        a[k] = a[k] + (0x9E3779B9u + (uint32_t)j);
+        a[k] = a[k] ^ (0x85EBCA6Bu + (uint32_t)(8 * j + k));
+      }
+    }
+  }
+  uint32_t r = 0;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) r ^= a[k];
+  out[g] = r;
+}
+
 // One lane per 16-byte chunk of the concatenated spans; the span of a chunk
 // is found by binary search over first_chunk.  A chunk moves as one 16-byte
 // word when source and destination are 16-byte aligned, else as 4-byte words.
@@ -937,12 +1017,13 @@ __global__ __launch_bounds__(256) void feedback_export_kernel(FeedbackExport x) 
 // it: it uses the device address noalloc_scratch looks up.
 static __device__ __align__(256) TraceFrame scratch[kNoAllocMaxViews];
 
-using TraceFn = void (*)(const TraceFrame*, uint32_t, uint32_t);
+using TraceFn = void (*)(const TraceFrame*, uint32_t, uint32_t, uint32_t);
 
 template <int POLICY, int TW, int WAVES>
 TraceFn pick_stack(uint32_t stack) {
-  return stack == SVO_STACK_PRIVATE ? trace_kernel<POLICY, TW, WAVES, SVO_STACK_PRIVATE>
-                                    : trace_kernel<POLICY, TW, WAVES, SVO_STACK_LDS>;
+  return stack == SVO_STACK_PRIVATE    ? trace_kernel<POLICY, TW, WAVES, SVO_STACK_PRIVATE>
+         : stack == SVO_STACK_LDS_NODE ? trace_kernel<POLICY, TW, WAVES, SVO_STACK_LDS_NODE>
+                                       : trace_kernel<POLICY, TW, WAVES, SVO_STACK_LDS>;
 }
 
 template <int POLICY, int TW>
@@ -984,7 +1065,7 @@ int noalloc_scratch(int device, TraceFrame** ptr) {
 
 bool shape_supported(const LaunchShape& s) {
   return (s.tile_w == 8 || s.tile_w == 16 || s.tile_w == 32) && (s.waves == 1 || s.waves == 2 || s.waves == 4) &&
-         (s.stack == SVO_STACK_LDS || s.stack == SVO_STACK_PRIVATE);
+         (s.stack == SVO_STACK_LDS || s.stack == SVO_STACK_PRIVATE || s.stack == SVO_STACK_LDS_NODE);
 }
 
 int launch_copy(void* dst, const void* src, size_t bytes, void* stream) {
@@ -993,6 +1074,13 @@ int launch_copy(void* dst, const void* src, size_t bytes, void* stream) {
   const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 256 * 64);
   (void)hipGetLastError();  // see launch_trace
   hipLaunchKernelGGL(copy_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream, (uint4*)dst, (const uint4*)src, n);
+  return (int)hipGetLastError();
+}
+
+int launch_valu_probe(uint32_t* out, uint32_t blocks, uint32_t iters, void* stream) {
+  if (blocks == 0) return 0;
+  (void)hipGetLastError();  // see launch_trace
+  hipLaunchKernelGGL(valu_probe_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream, out, iters);
   return (int)hipGetLastError();
 }
 
@@ -1020,11 +1108,15 @@ int launch_trace(const LaunchShape& shape, const TraceFrame* frames_dev, uint32_
                                                    : pick_tile<0>(shape);
   const uint32_t block = 64u * shape.waves;
   const dim3 grid((num_tiles + shape.waves - 1) / shape.waves);
-  const size_t lds = shape.stack == SVO_STACK_LDS ? (size_t)stack_levels * block * sizeof(uint2) : 0;
+  // LDS stack: 8 bytes per level and lane, plus 2 bytes of masks for the
+  // node-holding layout
+  const size_t lds = shape.stack == SVO_STACK_LDS        ? (size_t)stack_levels * block * sizeof(uint2)
+                     : shape.stack == SVO_STACK_LDS_NODE ? (size_t)stack_levels * block * (sizeof(uint2) + 2)
+                                                         : 0;
   // hipGetLastError reports the thread's last error from any earlier call:
   // clear it so that the check below sees this launch only
   (void)hipGetLastError();
-  hipLaunchKernelGGL(fn, grid, dim3(block), lds, (hipStream_t)stream, frames_dev, num_frames, num_tiles);
+  hipLaunchKernelGGL(fn, grid, dim3(block), lds, (hipStream_t)stream, frames_dev, num_frames, num_tiles, stack_levels);
   int e = (int)hipGetLastError();
   if (e != 0 || !shade) return e;
   const dim3 sgrid(num_tiles), sblock(64);

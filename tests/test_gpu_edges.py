@@ -224,10 +224,18 @@ def test_far_camera_maxloops_rays(tmp_path, policy):
     eye = eye / np.linalg.norm(eye) * 2000.0
     cam = svo.with_view_mode(svo.look_at(tuple(eye), (0.01, 0.02, 0.03), (0.0, 1.0, 0.0), 0.09, 64, 64,
                                          far=4000.0), "loops")
-    gpu = svo.render(mm, cam, policy, shade=True)
     ref = ora.render(cam, policy, shade=True)
-    _same(gpu, ref, f"far camera {policy}")
-    assert np.array_equal(gpu["pushes"], ref["pushes"])
-    assert np.array_equal(gpu["shade"].view(np.uint32), ref["shade"].view(np.uint32))
+    # every stack layout (the node-holding one finds the node a maxLoops ray
+    # reports by walking its path from the root)
+    for stack in ("lds", "lds_node", "private"):
+        r = svo.Renderer(0)
+        r.config = (8, 1, stack)
+        f = svo.Frame.alloc(cam.width, cam.height, pushes=True, shade=True)
+        r.render([svo.View(mm, cam, f)], policy)
+        svo._capi.check(svo.lib.svo_device_synchronize(), "sync")
+        gpu = f.fetch()
+        _same(gpu, ref, f"far camera {policy} {stack}")
+        assert np.array_equal(gpu["pushes"], ref["pushes"]), stack
+        assert np.array_equal(gpu["shade"].view(np.uint32), ref["shade"].view(np.uint32)), stack
     max_loops = 14 * 14 * 5 if policy == "treelet" else 7 * 7 * 5
     assert int((ref["loops"] == max_loops).sum()) > 40

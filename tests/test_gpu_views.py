@@ -131,7 +131,7 @@ def test_bad_launch_config_rejected():
 def test_autotune_installs_a_shape_with_identical_results(scenes):
     views = _views(scenes, seed=90)
     r = svo.Renderer(0)
-    _say("autotune over 18 shapes")
+    _say(f"autotune over {len(svo.LAUNCH_SHAPES)} shapes")
     shape, ms = r.autotune(views, "treelet", iters=2)
     _say(f"autotune picked {shape} at {ms:.4f} ms")
     assert shape in svo.LAUNCH_SHAPES and ms > 0

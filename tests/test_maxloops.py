@@ -211,3 +211,97 @@ def test_maxloops_rays_match_python_restatement(far_setup, policy):
         assert all(ref["node"][y, x] != MISS for (y, x) in stuck)
     else:  # and the CPU caster a miss
         assert all(ref["node"][y, x] == MISS for (y, x) in stuck)
+
+
+def _inner_nodes(nodes):
+    """Every inner node reachable from the root of a stitched incore buffer
+    (relative child offsets): index -> (tree depth, centre), in float64.  The
+    root is centred at the origin with edge 1; child c of a node of depth k
+    lies at centre + (bit - 0.5) * 2^-(k+1) per axis (x = bit 2)."""
+    out = {0: (0, np.zeros(3))}
+    stack = [0]
+    while stack:
+        i = stack.pop()
+        k, c = out[i]
+        first, masks = int(nodes[i, 0]), int(nodes[i, 1])
+        for ci in range(8):
+            if not (masks >> (8 + ci)) & 1 or (masks >> ci) & 1:
+                continue  # empty or leaf
+            j = (i + nth_child(masks, first, ci)) & 0xFFFFFFFF
+            bits = np.array([(ci >> 2) & 1, (ci >> 1) & 1, ci & 1], np.float64)
+            out[j] = (k + 1, c + (bits - 0.5) * 2.0 ** -(k + 1))
+            stack.append(j)
+    return out
+
+
+def _check_maxloops_hits(ref, dirs, origin, inner, stuck, max_loops):
+    """Geometric checks of the policy-A maxLoops result, independent of the
+    traversal's arithmetic: the reported node is an inner node of the tree,
+    reported with its own depth (the depth of its children, as sample()
+    counts scales) and centre; the point o + t*d lies in its box and on one
+    of its midplanes or faces (a stalled step sits on a boundary of the
+    child it keeps selecting), and the loop count is the closed form
+    (scaleMax+1)^2 * 5.
+    Returns the list of failures."""
+    bad = []
+    for (y, x) in stuck:
+        n = int(ref["node"][y, x])
+        where = f"pixel ({x}, {y})"
+        if int(ref["loops"][y, x]) != max_loops:
+            bad.append(f"{where}: loops {int(ref['loops'][y, x])} != {max_loops}")
+        if n not in inner:
+            bad.append(f"{where}: node {n} is not an inner node")
+            continue
+        k, c = inner[n]
+        if int(ref["depth"][y, x]) != k + 1:
+            bad.append(f"{where}: depth {int(ref['depth'][y, x])}, node depth {k}")
+        if not np.array_equal(ref["center"][y, x].astype(np.float64), c):
+            bad.append(f"{where}: centre {ref['center'][y, x]} != {c}")
+        p = origin + float(ref["t"][y, x]) * dirs[y, x].astype(np.float64)
+        half = 2.0 ** -(k + 1)
+        tol = 1e-3 * half + 1e-3  # t ~ 2000 in float32: p is good to ~2.5e-4
+        if np.any(np.abs(p - c) > half + tol):
+            bad.append(f"{where}: o + t d = {p} outside the box of node {n}")
+        rel = np.abs(p - c)  # a stalled step sits on a midplane or a face of the node
+        if np.min(np.minimum(rel, np.abs(rel - half))) > tol:
+            bad.append(f"{where}: o + t d = {p} on no plane of node {n}")
+    return bad
+
+
+def test_maxloops_hit_is_the_stalled_parent(far_setup):
+    """Policy A reports the parent node at maxLoops
+    (opencl/treeletRenderer_rgba.cl:257,268-277).  On every far-camera ray
+    that reaches maxLoops the oracle's result passes the geometric checks;
+    planting a wrong node (the reported node's parent, or a sibling) or an
+    off-by-one loop count in the oracle's result fails them."""
+    import oracle
+    sc, ora, nodes, cam = far_setup
+    ref = ora.render(cam, "treelet")
+    dirs = oracle.ray_dirs(cam)
+    origin = np.array(cam.position[:3], np.float64)
+    max_loops = 14 * 14 * 5  # scaleMax 13 for depth <= 12
+    stuck = [tuple(p) for p in np.argwhere(ref["loops"] == max_loops)]
+    assert len(stuck) > 40
+    inner = _inner_nodes(nodes)
+    assert not _check_maxloops_hits(ref, dirs, origin, inner, stuck, max_loops)
+    parent_of = {}
+    for i, (k, c) in inner.items():
+        first, masks = int(nodes[i, 0]), int(nodes[i, 1])
+        for ci in range(8):
+            if (masks >> (8 + ci)) & 1 and not (masks >> ci) & 1:
+                parent_of[(i + nth_child(masks, first, ci)) & 0xFFFFFFFF] = i
+    y, x = stuck[0]
+    for plant in ("parent", "sibling", "loops"):
+        bad = {k: v.copy() for k, v in ref.items()}
+        for (yy, xx) in stuck:
+            n = int(ref["node"][yy, xx])
+            if plant == "parent":
+                bad["node"][yy, xx] = parent_of[n]
+            elif plant == "sibling":
+                p = parent_of[n]
+                sibs = [j for j, q in parent_of.items() if q == p and j != n]
+                bad["node"][yy, xx] = sibs[0] if sibs else parent_of[n]
+            else:
+                bad["loops"][yy, xx] = max_loops - 1
+        fails = _check_maxloops_hits(bad, dirs, origin, inner, stuck, max_loops)
+        assert len(fails) >= len(stuck), f"planted {plant}: only {len(fails)} of {len(stuck)} rays caught"
