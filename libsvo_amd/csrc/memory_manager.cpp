@@ -1007,9 +1007,16 @@ svo_status mm_trace_pools(const svo_mm* mm, int policy, TracePools* P) {
     return SVO_ERR_ARG;
   }
   if (P->scale_max > 16) { set_error("scene depth %u exceeds the 16-level traversal stack", D); return SVO_ERR_ARG; }
-  // a push leaves scale s for s-1; policy A never pushes from scale
-  // scale_max - lod_cap (the depth cap stops there), policy C never from 0
-  P->min_push_scale = P->lod_cap != 0 ? P->scale_max - P->lod_cap + 1 : 1;
+  // A push leaves scale s for s-1 and makes the child at depth scale_max - s
+  // the current node.  Policy A never pushes from scale scale_max - lod_cap
+  // (the depth cap stops there), policy C never from 0; and no policy pushes
+  // a node deeper than the scene's deepest pushable node (inner_depth), i.e.
+  // from a scale below scale_max - inner_depth (a well-formed tree of depth
+  // D pushes nodes down to depth D - 1: one stack level less than D).
+  uint32_t mps = P->lod_cap != 0 ? P->scale_max - P->lod_cap + 1 : 1;
+  const uint32_t inner = mm->scene->inner_depth;
+  if (inner < P->scale_max && P->scale_max - inner > mps) mps = P->scale_max - inner;
+  P->min_push_scale = mps;
   P->stack_levels = P->scale_max > P->min_push_scale ? P->scale_max - P->min_push_scale : 1;
   return SVO_OK;
 }

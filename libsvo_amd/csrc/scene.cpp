@@ -355,6 +355,7 @@ svo_status build_scene(uint32_t depth, std::vector<uint64_t>& codes,
   sc->treelet_bytes = tb;
   sc->nodes_per_treelet = N;
   sc->max_depth = depth;
+  sc->inner_depth = depth > 0 ? depth - 1 : 0;  // every leaf lies at `depth`
   sc->tree_nodes = tree_nodes;
   sc->tree_leaves = leaves;
 
@@ -459,7 +460,7 @@ bool read_u32(FILE* f, uint32_t* v) { return fread(v, 4, 1, f) == 1; }
 // Depth of every node of every treelet, checking that child offsets stay
 // inside the treelet.  Returns the depth of the deepest valid node (the
 // deepest leaf in a well-formed tree), or -1 on a bad offset.
-int scene_depth(const svo_scene& sc, std::string* why) {
+int scene_depth(const svo_scene& sc, std::string* why, uint32_t* inner_depth) {
   const size_t nt = sc.treelets.size();
   std::vector<uint32_t> root_depth(nt, 0);
   // children of each treelet in CSR form (parent gid < child gid)
@@ -471,11 +472,13 @@ int scene_depth(const svo_scene& sc, std::string* why) {
     for (size_t u = 1; u < nt; ++u) clist[fill[sc.treelets[u].h.parent_gid]++] = (uint32_t)u;
   }
   int maxd = 0;
+  uint32_t inner = 0;
   std::vector<uint32_t> nd;
   for (size_t t = 0; t < nt; ++t) {
     const TreeletData& T = sc.treelets[t];
     nd.assign(T.h.num_nodes, 0);
     nd[0] = root_depth[t];
+    if (T.h.num_nodes != 0 && nd[0] > inner) inner = nd[0];  // a (sub-)treelet root
     // BFS order: a node's children have larger local indices.
     for (uint32_t i = 0; i < T.h.num_nodes; ++i) {
       const svo_node& n = T.nodes[i];
@@ -496,6 +499,7 @@ int scene_depth(const svo_scene& sc, std::string* why) {
         // every valid node counts, not only leaves: an inner node below the
         // deepest leaf (a malformed tree) must still fit the traversal stack
         if (nd[ci] > (uint32_t)maxd) maxd = (int)nd[ci];
+        if (!((n.masks >> c) & 1u) && nd[ci] > inner) inner = nd[ci];  // a valid non-leaf child
       }
     }
     // children treelets of t take their root depth from t's leaf
@@ -504,6 +508,7 @@ int scene_depth(const svo_scene& sc, std::string* why) {
       root_depth[u] = nd[sc.treelets[u].h.parent_leaf_pos];
     }
   }
+  if (inner_depth) *inner_depth = inner;
   return maxd;
 }
 
@@ -589,7 +594,7 @@ extern "C" svo_status svo_scene_open(const char* base, svo_scene** out) {
     }
   }
   std::string why;
-  int d = scene_depth(*sc, &why);
+  int d = scene_depth(*sc, &why, &sc->inner_depth);
   if (d < 0) { set_error("%s: %s", p.c_str(), why.c_str()); return SVO_ERR_FORMAT; }
   sc->max_depth = (uint32_t)d;
   uint64_t nodes = 0, leaves = 0;

@@ -127,7 +127,7 @@ extern "C" svo_status svo_scene_open_single(const char* base, svo_scene** out) {
   sc->nodes_per_treelet = (uint32_t)(tb / 8);
   sc->treelets.push_back(std::move(T));
   std::string why;
-  const int d = scene_depth(*sc, &why);
+  const int d = scene_depth(*sc, &why, &sc->inner_depth);
   if (d < 0) { set_error("%s: %s", p.c_str(), why.c_str()); return SVO_ERR_FORMAT; }
   if ((uint32_t)d != depth) { set_error("%s: header depth %u, tree depth %d", p.c_str(), depth, d); return SVO_ERR_FORMAT; }
   sc->max_depth = (uint32_t)d;

@@ -72,6 +72,10 @@ struct svo_scene {
   uint32_t treelet_bytes = 0;
   uint32_t nodes_per_treelet = 0;
   uint32_t max_depth = 0;
+  // depth of the deepest node a traversal can push: a valid non-leaf node
+  // or a treelet root (an anchor leaf becomes one when its sub-treelet is
+  // stitched in); max_depth - 1 in a well-formed tree
+  uint32_t inner_depth = 0;
   uint64_t tree_nodes = 0;
   uint64_t tree_leaves = 0;
   std::vector<svo_impl::TreeletData> treelets;
@@ -82,7 +86,7 @@ namespace svo_impl {
 // Depth of the deepest valid node of a scene (root = 0; the deepest leaf in a
 // well-formed tree), checking that every child offset stays inside its
 // treelet; -1 (and *why) on a bad offset.  scene.cpp.
-int scene_depth(const svo_scene& sc, std::string* why);
+int scene_depth(const svo_scene& sc, std::string* why, uint32_t* inner_depth = nullptr);
 
 struct TracePools;
 

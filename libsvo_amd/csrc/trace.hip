@@ -341,10 +341,12 @@ __device__ Hit traverse(const TracePools& S, const float o[3], float d[3], float
           hit_kind = is_leaf ? 1u : 2u;
           hit_ci = ci;
           hit_tcmin = tc_min;
-        } else if (child >= S.pool_nodes || (POLICY == 1 && scale < mps)) {
+        } else if (child >= S.pool_nodes || scale < mps) {
           // corrupt offset (never read outside the pool), or a push below
-          // the stack: policy 1 has no depth stop, so an inner node deeper
-          // than the stack holds (a malformed tree) ends the ray here
+          // the stack: an inner node deeper than the scene's deepest
+          // pushable node, or (policy 1, which has no depth stop) deeper
+          // than its 15 levels, can only come from a malformed tree; the ray
+          // ends here, and the stack is never written below level 0
           outcome = 4;
         } else {  // push
           ++pushes;
@@ -565,8 +567,8 @@ __device__ Hit traverse_fast(const Uniform& U, const float o[3], const float d[3
             hit_kind = is_leaf ? 1u : 2u;
             hit_ci = ci;
             hit_err = is_leaf ? leaf_error(se, ratio, tc_min) : stop_error(se, ratio, tc_min);
-          } else if (child >= U.pool_nodes || (POLICY == 1 && scale < U.mps)) {
-            outcome = 4;  // corrupt offset, or (policy 1) a push below the stack
+          } else if (child >= U.pool_nodes || scale < U.mps) {
+            outcome = 4;  // corrupt offset, or a push below the stack
           } else {
             ++pushes;
             if constexpr (Stack::kHoldsNodes) stk.put3(scale - U.mps, nidx + (uint32_t)nd, masks, tmax);
@@ -789,8 +791,12 @@ __device__ float4 shade_pixel(uint32_t mode, const Hit& hit, const float d[3], c
 }
 
 template <int POLICY, int TW, int WAVES, int STACK>
-__global__ __launch_bounds__(64 * WAVES) void trace_kernel(const TraceFrame* __restrict__ frames, uint32_t num_frames,
-                                                           uint32_t num_tiles, uint32_t stack_levels) {
+// Eight waves per SIMD: the register budget is held to 64 VGPRs (a few
+// spills outside the inner loop), and a 10-level LDS stack (5 KiB a wave)
+// lets 32 waves share a CU's 160 KiB.
+__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(8, 8)))
+void trace_kernel(const TraceFrame* __restrict__ frames, uint32_t num_frames, uint32_t num_tiles,
+                  uint32_t stack_levels) {
   constexpr int kBlock = 64 * WAVES;
   constexpr uint32_t TH = 64u / TW;
   extern __shared__ uint32_t s_stack[];
