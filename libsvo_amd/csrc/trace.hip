@@ -465,6 +465,7 @@ __device__ Hit traverse_fast(const Uniform& U, const float o[3], const float d[3
   uint64_t path = 0;
   uint32_t iter = 0;
   uint32_t trip = 0;  // loop trips of the wave (uniform); schedules the cycle checks
+  uint32_t next_check = 128u;
   uint32_t pushes = 0;
   uint64_t snap_path = 0;
   uint32_t snap_iter = 0, snap_scale = 0, snap_tmin = 0, snap_tmax = 0, snap_pushes = 0;
@@ -480,7 +481,8 @@ __device__ Hit traverse_fast(const Uniform& U, const float o[3], const float d[3
     ++trip;
     // snapshots at power-of-two trips >= 128, comparisons every 8th trip: a
     // repeat found at any multiple of the period is a period too
-    if (trip >= 128u && (trip & 7u) == 0u) {
+    if (trip == next_check) {
+      next_check += 8u;
       if ((trip & (trip - 1u)) == 0u) {
         snap_iter = iter; snap_scale = (uint32_t)scale; snap_path = path;
         snap_tmin = __float_as_uint(tmin); snap_tmax = __float_as_uint(tmax); snap_pushes = pushes;
