@@ -318,7 +318,7 @@ def _ooc_flight(mm, cams, policy, divide, renderer, stream, fused=False):
         renderer.render(view, policy, stream)
         stream.synchronize()
         d = time.perf_counter()
-        t_feedback += b - a
+        t_feedback += b - a2  # readback and host update, without the analyse pass
         t_analyse += a2 - a
         t_upload += c - b
         t_render += d - c
