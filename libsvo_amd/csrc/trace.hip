@@ -4,10 +4,14 @@
 // wavefronts per workgroup; every view of a launch may use its own scene.
 // The per-ray stack holds, per level, the node index of the parent left by a
 // push and that parent's tmax (8 bytes), in LDS ([level][lane],
-// conflict-free) or in private memory (the launch shape picks; the autotuner
-// measures both).  The child index taken from each parent goes to a 64-bit
-// path register (3 bits per level); the parent node itself is re-read on a
-// pop (a cache hit: it was fetched a few iterations earlier).
+// conflict-free) or in private memory, or the parent's child base, tmax and
+// 16-bit masks (10 bytes, LDS) so that a pop reads no global memory (the
+// launch shape picks; the autotuner measures all three).  The child index
+// taken from each parent goes to a 64-bit path register (3 bits per level);
+// with the 8-byte layouts the parent node itself is re-read on a pop (a
+// cache hit: it was fetched a few iterations earlier).  The stack has a
+// level for every scale a push can leave, down to the scene's deepest
+// pushable node.
 // Everything else about a parent is recomputed bit-exactly instead of stored:
 //   * centres are dyadic, so parent = child - (-0.5 + bit) * 2^(s - scaleMax)
 //     is exact;
