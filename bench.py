@@ -27,7 +27,12 @@ Also reported in the same line:
 * ``serving``: Poisson request streams over four resident scenes (Menger
   depth 9, terrain depth 11, sphere cloud depth 10, terrain depth 12) in a
   64 GB HBM budget, served by the continuous-batching server and by the plain
-  one-frame-per-launch loop: frames/s, Mrays/s, p50/p99 latency.
+  one-frame-per-launch loop: frames/s, Mrays/s, p50/p99 latency, from low
+  rates past saturation;
+* ``autotune``: the headline's autotune cache key, the tuning time and the
+  time of a second tuning call of the same workload class (a cache hit);
+* ``cpu_baseline``, ``boundary``: the CPU caster's restatement on the host
+  and the headline frame's boundary-ray and differing-pixel counts.
 
 Prints one JSON line (rank 0, one GPU).  Progress goes to stderr.
 """
@@ -577,8 +582,17 @@ def main():
         shape = renderer.config
         log(f"launch shape {shape} (given)")
     else:
-        shape, tune_ms = renderer.autotune(view, args.policy, stream, iters=20)  # shapes lie within a few %
-        log(f"autotuned launch shape {shape} ({tune_ms:.4f} ms in tuning)")
+        tune_cache = svo.TuneCache()
+        t0 = time.perf_counter()
+        shape, tune_ms = renderer.autotune(view, args.policy, stream, iters=20, cache=tune_cache)  # shapes lie within a few %
+        t1 = time.perf_counter()
+        renderer.autotune(view, args.policy, stream, iters=20, cache=tune_cache)  # the same class again: a cache hit
+        t2 = time.perf_counter()
+        autotune = {"key": renderer.tune_key(view, args.policy), "shape": list(shape),
+                    "tuning_s": round(t1 - t0, 3), "cached_lookup_ms": round((t2 - t1) * 1e3, 3),
+                    "second_call_hit": tune_cache.last_hit}
+        log(f"autotuned launch shape {shape} ({tune_ms:.4f} ms in tuning, {t1 - t0:.2f} s); cache hit "
+            f"{tune_cache.last_hit} in {(t2 - t1) * 1e3:.2f} ms")
     if args.warmup > 0:
         renderer.timed(view, args.policy, stream, iters=args.warmup)
     svo._capi.check(svo.lib.svo_device_synchronize(), "sync")
@@ -652,6 +666,8 @@ def main():
             "issue": profiled_issue(shape),
         },
     }
+    if not args.launch_shape:
+        out["autotune"] = autotune
     # both policies' frames (untimed) for the CPU leg's pixel check
     gpu_frames = {}
     for pol in ("treelet", "cpu3"):

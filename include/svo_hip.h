@@ -518,6 +518,39 @@ svo_status svo_autotune(svo_renderer* r, const svo_view* views, uint32_t n,
                         int policy, void* stream, uint32_t iters,
                         svo_launch_config* best, float* best_ms);
 
+/* Autotune cache: the shapes svo_autotune picked, kept per workload class so
+ * that a process, or the next one through a file, does not time the 27
+ * shapes again.  No counterpart in the reference, whose launch shape is the
+ * fixed work size of the renderToBuffer enqueue
+ * (raycast_serialized_opencl.cpp:729-749) and whose stack depth is a
+ * compile-time constant (opencl/treeletRenderer_rgba.cl:7-8).
+ * The key (svo_tune_cache_key) names the device architecture and CU count,
+ * the policy, the number of views and the total rays (each rounded up to a
+ * power of two), and every distinct scene class of the views: scaleMax,
+ * stack levels, nodes per treelet, LOD on or off.
+ * File: text, one entry per line, `key<TAB>tile_width<TAB>waves<TAB>stack
+ * <TAB>ms` with stack lds / private / lds_node; lines starting with '#' are
+ * comments.  svo_tune_cache_load merges a file into the cache (a missing file
+ * is an empty cache, a malformed line SVO_ERR_FORMAT and nothing merged);
+ * svo_tune_cache_save writes the whole cache through a temporary file and a
+ * rename.  A cache may be shared by renderers and threads.
+ * svo_autotune_cached sets the renderer's shape from the cache when the key
+ * is there (*cache_hit = 1, *best_ms = the time recorded with it), otherwise
+ * runs svo_autotune and records its pick (*cache_hit = 0). */
+typedef struct svo_tune_cache svo_tune_cache;
+svo_status svo_tune_cache_create(svo_tune_cache** out);
+void svo_tune_cache_destroy(svo_tune_cache* c);
+uint32_t svo_tune_cache_size(const svo_tune_cache* c);
+svo_status svo_tune_cache_load(svo_tune_cache* c, const char* path);
+svo_status svo_tune_cache_save(svo_tune_cache* c, const char* path);
+svo_status svo_tune_cache_key(svo_renderer* r, const svo_view* views, uint32_t n,
+                              int policy, char* key, size_t cap);
+svo_status svo_autotune_cached(svo_renderer* r, svo_tune_cache* c,
+                               const svo_view* views, uint32_t n, int policy,
+                               void* stream, uint32_t iters,
+                               svo_launch_config* best, float* best_ms,
+                               int* cache_hit);
+
 /* A batch is a multi-view launch captured once into a hipGraph and replayed
  * with svo_batch_launch.  Its view descriptors live in device memory owned by
  * the batch; svo_batch_set_cameras rewrites the camera uniforms (same frame

@@ -16,6 +16,7 @@ libsvo reads the same:
 from __future__ import annotations
 
 import ctypes
+import os
 from dataclasses import dataclass
 from typing import Dict, List, Optional, Sequence, Tuple
 
@@ -27,7 +28,7 @@ from ._capi import (BACKGROUND_RGBA, MISS, SVO_BUILD_PACK_BFS, SVO_POLICY_CPU3, 
 
 __all__ = [
     "Scene", "TreeletMemoryManagerHip", "Camera", "DeviceBuffer", "Stream", "Frame", "render_to_buffer",
-    "View", "Renderer", "Batch", "LaunchConfig", "LAUNCH_SHAPES", "Server",
+    "View", "Renderer", "TuneCache", "Batch", "LaunchConfig", "LAUNCH_SHAPES", "Server",
     "device_count", "look_at", "SvoError", "POLICIES", "MISS", "BACKGROUND_RGBA",
 ]
 
@@ -612,14 +613,67 @@ class Renderer:
               "svo_render_views_timed")
         return ms.value
 
-    def autotune(self, views: Sequence[View], policy="treelet", stream: Optional[Stream] = None, iters: int = 3):
+    def autotune(self, views: Sequence[View], policy="treelet", stream: Optional[Stream] = None, iters: int = 3,
+                 cache: Optional["TuneCache"] = None):
         """Times every launch shape on these views, keeps the fastest and
-        returns ((tile_width, waves, stack), ms)."""
+        returns ((tile_width, waves, stack), ms).  With a :class:`TuneCache`
+        a pick recorded for the same workload class is taken without timing
+        (``cache.last_hit`` tells which happened) and a new pick is
+        recorded."""
         best, ms = svo_launch_config(), ctypes.c_float()
+        if cache is not None:
+            hit = ctypes.c_int()
+            check(lib.svo_autotune_cached(self._h, cache.handle, _view_array(views), len(views), _policy(policy),
+                                          stream.handle if stream else None, iters, ctypes.byref(best),
+                                          ctypes.byref(ms), ctypes.byref(hit)), "svo_autotune_cached")
+            cache.last_hit = bool(hit.value)
+            return self.config, ms.value
         check(lib.svo_autotune(self._h, _view_array(views), len(views), _policy(policy),
                                stream.handle if stream else None, iters, ctypes.byref(best), ctypes.byref(ms)),
               "svo_autotune")
         return self.config, ms.value
+
+    def tune_key(self, views: Sequence[View], policy="treelet") -> str:
+        """The autotune cache key of these views on this renderer's device."""
+        buf = ctypes.create_string_buffer(4096)
+        check(lib.svo_tune_cache_key(self._h, _view_array(views), len(views), _policy(policy), buf, len(buf)),
+              "svo_tune_cache_key")
+        return buf.value.decode()
+
+
+class TuneCache:
+    """Launch shapes picked by :meth:`Renderer.autotune`, per workload class
+    (device architecture and CU count, policy, number of views and rays,
+    scene classes), optionally kept in a text file between processes."""
+
+    def __init__(self, path: Optional[str] = None):
+        self._h = ctypes.c_void_p()
+        check(lib.svo_tune_cache_create(ctypes.byref(self._h)), "svo_tune_cache_create")
+        self.path = path
+        self.last_hit = False
+        if path is not None:
+            self.load(path)
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib.svo_tune_cache_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    @property
+    def handle(self) -> ctypes.c_void_p:
+        return self._h
+
+    def __len__(self) -> int:
+        return int(lib.svo_tune_cache_size(self._h))
+
+    def load(self, path: str) -> None:
+        check(lib.svo_tune_cache_load(self._h, os.fsencode(path)), "svo_tune_cache_load")
+
+    def save(self, path: Optional[str] = None) -> None:
+        path = path or self.path
+        if path is None:
+            raise ValueError("TuneCache.save: no path")
+        check(lib.svo_tune_cache_save(self._h, os.fsencode(path)), "svo_tune_cache_save")
 
 
 class Batch:

@@ -159,6 +159,14 @@ SIGNATURES = {
     "svo_render_views_noalloc": (ctypes.c_int, [P(svo_view), u32, ctypes.c_int, P(svo_launch_config), vp]),
     "svo_render_views_timed": (ctypes.c_int, [vp, P(svo_view), u32, ctypes.c_int, vp, u32, P(f32)]),
     "svo_autotune": (ctypes.c_int, [vp, P(svo_view), u32, ctypes.c_int, vp, u32, P(svo_launch_config), P(f32)]),
+    "svo_tune_cache_create": (ctypes.c_int, [P(vp)]),
+    "svo_tune_cache_destroy": (None, [vp]),
+    "svo_tune_cache_size": (u32, [vp]),
+    "svo_tune_cache_load": (ctypes.c_int, [vp, ctypes.c_char_p]),
+    "svo_tune_cache_save": (ctypes.c_int, [vp, ctypes.c_char_p]),
+    "svo_tune_cache_key": (ctypes.c_int, [vp, P(svo_view), u32, ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t]),
+    "svo_autotune_cached": (ctypes.c_int, [vp, vp, P(svo_view), u32, ctypes.c_int, vp, u32, P(svo_launch_config),
+                                            P(f32), P(ctypes.c_int)]),
     "svo_batch_create": (ctypes.c_int, [vp, P(svo_view), u32, ctypes.c_int, vp, P(vp)]),
     "svo_batch_set_cameras": (ctypes.c_int, [vp, P(svo_camera), u32, vp]),
     "svo_batch_launch": (ctypes.c_int, [vp, vp]),

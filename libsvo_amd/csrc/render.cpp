@@ -8,8 +8,12 @@
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <map>
 #include <memory>
 #include <mutex>
+#include <string>
 #include <vector>
 
 #include "svo_internal.h"
@@ -279,6 +283,172 @@ extern "C" svo_status svo_autotune(svo_renderer* r, const svo_view* views, uint3
   r->shape = bs;
   if (best) *best = svo_launch_config{bs.tile_w, bs.waves, bs.stack, 0};
   if (best_ms) *best_ms = bm;
+  return SVO_OK;
+}
+
+// ------------------------------------------------------------ autotune cache
+
+namespace {
+const char* const kStackNames[] = {"lds", "private", "lds_node"};
+const char kCacheHeader[] = "# libsvo_amd autotune cache v1: key<TAB>tile_width<TAB>waves<TAB>stack<TAB>ms";
+
+uint64_t pow2_at_least(uint64_t v) {
+  uint64_t p = 1;
+  while (p < v) p <<= 1;
+  return p;
+}
+}  // namespace
+
+struct svo_tune_cache {
+  struct Entry {
+    LaunchShape shape;
+    float ms;
+  };
+  std::mutex lock;
+  std::map<std::string, Entry> entries;
+};
+
+extern "C" svo_status svo_tune_cache_create(svo_tune_cache** out) {
+  if (!out) { set_error("svo_tune_cache_create: null out"); return SVO_ERR_ARG; }
+  *out = new svo_tune_cache();
+  return SVO_OK;
+}
+
+extern "C" void svo_tune_cache_destroy(svo_tune_cache* c) { delete c; }
+
+extern "C" uint32_t svo_tune_cache_size(const svo_tune_cache* c) {
+  if (!c) return 0;
+  std::lock_guard<std::mutex> g(const_cast<svo_tune_cache*>(c)->lock);
+  return (uint32_t)c->entries.size();
+}
+
+extern "C" svo_status svo_tune_cache_load(svo_tune_cache* c, const char* path) {
+  if (!c || !path) { set_error("svo_tune_cache_load: bad arguments"); return SVO_ERR_ARG; }
+  FILE* f = fopen(path, "r");
+  if (!f) return SVO_OK;  // no file yet: nothing cached
+  std::map<std::string, svo_tune_cache::Entry> read;
+  char line[1024];
+  unsigned lineno = 0;
+  svo_status st = SVO_OK;
+  while (fgets(line, sizeof line, f)) {
+    ++lineno;
+    size_t len = strlen(line);
+    if (len == sizeof line - 1 && line[len - 1] != '\n') {
+      set_error("%s:%u: line too long", path, lineno);
+      st = SVO_ERR_FORMAT;
+      break;
+    }
+    while (len && (line[len - 1] == '\n' || line[len - 1] == '\r')) line[--len] = 0;
+    if (len == 0 || line[0] == '#') continue;
+    char* tab = strchr(line, '\t');
+    unsigned tw = 0, w = 0;
+    char stack[16] = {0};
+    float ms = 0;
+    if (!tab || sscanf(tab + 1, "%u\t%u\t%15s\t%f", &tw, &w, stack, &ms) != 4) {
+      set_error("%s:%u: expected key, tile width, waves, stack and ms separated by tabs", path, lineno);
+      st = SVO_ERR_FORMAT;
+      break;
+    }
+    uint32_t k = 3;
+    for (uint32_t i = 0; i < 3; ++i)
+      if (strcmp(stack, kStackNames[i]) == 0) k = i;
+    const svo_launch_config cfg{tw, w, k, 0};
+    LaunchShape sh{};
+    if (k == 3 || shape_from_config(&cfg, &sh) != SVO_OK) {
+      set_error("%s:%u: no launch shape (%u, %u, %s)", path, lineno, tw, w, stack);
+      st = SVO_ERR_FORMAT;
+      break;
+    }
+    read[std::string(line, tab)] = svo_tune_cache::Entry{sh, ms};
+  }
+  fclose(f);
+  if (st != SVO_OK) return st;
+  std::lock_guard<std::mutex> g(c->lock);
+  for (auto& kv : read) c->entries[kv.first] = kv.second;
+  return SVO_OK;
+}
+
+extern "C" svo_status svo_tune_cache_save(svo_tune_cache* c, const char* path) {
+  if (!c || !path) { set_error("svo_tune_cache_save: bad arguments"); return SVO_ERR_ARG; }
+  const std::string tmp = std::string(path) + ".tmp";
+  FILE* f = fopen(tmp.c_str(), "w");
+  if (!f) { set_error("svo_tune_cache_save: cannot write %s", tmp.c_str()); return SVO_ERR_IO; }
+  bool ok = fprintf(f, "%s\n", kCacheHeader) > 0;
+  {
+    std::lock_guard<std::mutex> g(c->lock);
+    for (const auto& kv : c->entries)
+      ok = ok && fprintf(f, "%s\t%u\t%u\t%s\t%.5f\n", kv.first.c_str(), kv.second.shape.tile_w, kv.second.shape.waves,
+                         kStackNames[kv.second.shape.stack], kv.second.ms) > 0;
+  }
+  ok = (fclose(f) == 0) && ok;
+  if (!ok || rename(tmp.c_str(), path) != 0) {
+    remove(tmp.c_str());
+    set_error("svo_tune_cache_save: cannot write %s", path);
+    return SVO_ERR_IO;
+  }
+  return SVO_OK;
+}
+
+extern "C" svo_status svo_tune_cache_key(svo_renderer* r, const svo_view* views, uint32_t n, int policy, char* key,
+                                         size_t cap) {
+  if (!r || !views || n == 0 || !key || cap == 0) { set_error("svo_tune_cache_key: bad arguments"); return SVO_ERR_ARG; }
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, r->device));
+  std::string arch(prop.gcnArchName);
+  arch = arch.substr(0, arch.find(':'));
+  // one class per distinct scene shape: scaleMax, stack levels, nodes per
+  // treelet and whether the LOD stop is on
+  std::vector<std::string> scenes;
+  uint64_t rays = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    if (!views[i].mm) { set_error("view %u has no residency handle", i); return SVO_ERR_ARG; }
+    TracePools P;
+    const svo_status st = mm_trace_pools(views[i].mm, policy, &P);
+    if (st != SVO_OK) return st;
+    char s[96];
+    snprintf(s, sizeof s, "%u/%u/%u/%s", P.scale_max, P.stack_levels, P.nodes_per_treelet,
+             views[i].camera.t_scale_ratio != 0.0f ? "lod" : "full");
+    scenes.emplace_back(s);
+    rays += (uint64_t)views[i].camera.width * views[i].camera.height;
+  }
+  std::sort(scenes.begin(), scenes.end());
+  scenes.erase(std::unique(scenes.begin(), scenes.end()), scenes.end());
+  std::string k = "arch=" + arch + ";cus=" + std::to_string(prop.multiProcessorCount) +
+                  ";policy=" + std::to_string(policy) + ";views=" + std::to_string(pow2_at_least(n)) +
+                  ";rays=" + std::to_string(pow2_at_least(rays)) + ";scenes=";
+  for (size_t i = 0; i < scenes.size(); ++i) k += (i ? "," : "") + scenes[i];
+  if (k.size() + 1 > cap) { set_error("svo_tune_cache_key: key needs %zu bytes", k.size() + 1); return SVO_ERR_ARG; }
+  memcpy(key, k.c_str(), k.size() + 1);
+  return SVO_OK;
+}
+
+extern "C" svo_status svo_autotune_cached(svo_renderer* r, svo_tune_cache* c, const svo_view* views, uint32_t n,
+                                          int policy, void* stream, uint32_t iters, svo_launch_config* best,
+                                          float* best_ms, int* cache_hit) {
+  if (!r || !c) { set_error("svo_autotune_cached: bad arguments"); return SVO_ERR_ARG; }
+  char key[4096];
+  svo_status st = svo_tune_cache_key(r, views, n, policy, key, sizeof key);
+  if (st != SVO_OK) return st;
+  {
+    std::lock_guard<std::mutex> g(c->lock);
+    auto it = c->entries.find(key);
+    if (it != c->entries.end()) {
+      r->shape = it->second.shape;
+      if (best) *best = svo_launch_config{r->shape.tile_w, r->shape.waves, r->shape.stack, 0};
+      if (best_ms) *best_ms = it->second.ms;
+      if (cache_hit) *cache_hit = 1;
+      return SVO_OK;
+    }
+  }
+  float ms = 0;
+  st = svo_autotune(r, views, n, policy, stream, iters, best, &ms);
+  if (st != SVO_OK) return st;
+  {
+    std::lock_guard<std::mutex> g(c->lock);
+    c->entries[key] = svo_tune_cache::Entry{r->shape, ms};
+  }
+  if (best_ms) *best_ms = ms;
+  if (cache_hit) *cache_hit = 0;
   return SVO_OK;
 }
 

@@ -141,6 +141,38 @@ def test_autotune_installs_a_shape_with_identical_results(scenes):
         _compare(v.frame.fetch(), _oracle(v.mm, v.camera, "treelet"), f"tuned view {i}")
 
 
+def test_autotune_cache_hit_skips_timing_and_persists(scenes, tmp_path):
+    views = _views(scenes, seed=91)
+    r = svo.Renderer(0)
+    key = r.tune_key(views, "treelet")
+    _say(f"tune key {key}")
+    assert key.startswith("arch=gfx950;") and ";policy=0;" in key and ";views=" in key
+    cache = svo.TuneCache()
+    shape, ms = r.autotune(views, "treelet", iters=2, cache=cache)
+    assert not cache.last_hit and len(cache) == 1 and shape in svo.LAUNCH_SHAPES
+    # a hit sets the recorded shape on another renderer without timing
+    r2 = svo.Renderer(0)
+    r2.config = (32, 4, "private") if shape != (32, 4, "private") else (8, 1, "lds")
+    shape2, ms2 = r2.autotune(views, "treelet", iters=2, cache=cache)
+    assert cache.last_hit and shape2 == shape and ms2 == pytest.approx(ms)
+    # other views of another class miss
+    small = [svo.View(scenes["menger"], workloads.sphere_camera(3, 40, 24), svo.Frame.alloc(40, 24))]
+    assert r2.tune_key(small, "treelet") != key
+    r2.autotune(small, "treelet", iters=1, cache=cache)
+    assert not cache.last_hit and len(cache) == 2
+    # through a file, into a new process-wide cache
+    path = str(tmp_path / "tune.tsv")
+    cache.save(path)
+    fresh = svo.TuneCache(path)
+    r3 = svo.Renderer(0)
+    shape3, _ = r3.autotune(views, "treelet", iters=2, cache=fresh)
+    assert fresh.last_hit and shape3 == shape
+    r3.render(views, "treelet")
+    _sync()
+    for i, v in enumerate(views):
+        _compare(v.frame.fetch(), _oracle(v.mm, v.camera, "treelet"), f"cached-shape view {i}")
+
+
 def test_batch_graph_replays_and_camera_updates(scenes):
     views = _views(scenes, seed=120)
     r = svo.Renderer(0)
