@@ -222,10 +222,37 @@ typedef struct svo_camera {
  *              tScaleRatio*tcMin at a leaf, 0 at an LOD stop; miss
  *              0.6,0.3,0.3,1
  * The reference's FP64 literals in the phong and colour-map code are taken
- * as FP32 (that code does not build as OpenCL C without double support). */
+ * as FP32 (that code does not build as OpenCL C without double support).
+ *
+ * The single-SVO renderer's modes (opencl/fillFrameBuffer_rgba.cl:600-704,
+ * its otherParams.x 0..6 = COLOR, NORMAL, FILL_SVO_DEPTH, T, FILL_LOOPS,
+ * FILL_REFLECTION, FILL_SHADOW):
+ *   FILL_SVO_DEPTH   depth/12 on a hit, 0 on a miss (its shade_svoDepth)
+ *   FILL_LOOPS       1 - loops/300 (its shade_iterationDepth)
+ *   FILL_SHADOW      shade_diffuse_color_shadow (:494-530): colour * nDotL *
+ *                    (1 - shadow) + colour * 0.2, alpha 1, light direction
+ *                    normalize(-0.5, 1, 0.6); shadow = 1 when a policy-B ray
+ *                    from nodeCenter + 0.01 * normal towards the light hits
+ *                    (tScaleRatio 0.001; like every sample() it is traced
+ *                    from where its line enters the root box); miss
+ *                    0.2,0.3,0.2,1
+ *   FILL_REFLECTION  shade_diffuse_color_reflection (:536-594) with two
+ *                    reflections: the shadow shading of the hit, then of up
+ *                    to two reflection hits (origin nodeCenter + 0.01 *
+ *                    normal, direction reflected about the normal), blended
+ *                    with strengths 0.75, 0.5625, 0.421875.  The reference
+ *                    never looks the attribute index up in this mode (its
+ *                    case 5 leaves SampleResult.attribIndex at 0), so every
+ *                    bounce shades with attribute package 0: for a scene
+ *                    from a single-SVO file the file's package 0, otherwise
+ *                    the root node's attributes.
+ * FILL_SHADOW and FILL_REFLECTION trace their secondary rays with the
+ * single-SVO policy and need SVO_POLICY_SINGLE for the primary rays. */
 enum {
   SVO_VIEW_COLOR = 0, SVO_VIEW_NORMAL = 1, SVO_VIEW_ITERATIONS = 2, SVO_VIEW_SVO_DEPTH = 3,
-  SVO_VIEW_T = 4, SVO_VIEW_LOOPS = 5, SVO_VIEW_PHONG = 6, SVO_VIEW_QUALITY = 7
+  SVO_VIEW_T = 4, SVO_VIEW_LOOPS = 5, SVO_VIEW_PHONG = 6, SVO_VIEW_QUALITY = 7,
+  SVO_VIEW_FILL_SVO_DEPTH = 8, SVO_VIEW_FILL_LOOPS = 9, SVO_VIEW_FILL_REFLECTION = 10,
+  SVO_VIEW_FILL_SHADOW = 11
 };
 
 /* Pinhole camera at `eye` looking at `target` with `up`, vertical field of

@@ -28,7 +28,11 @@ SVO_BUILD_PACK_BFS = 1
 SVO_SINGLE_SVOLOADER, SVO_SINGLE_V1 = 1, 2
 SVO_STACK_LDS, SVO_STACK_PRIVATE, SVO_STACK_LDS_NODE = 0, 1, 2
 SVO_NOALLOC_MAX_VIEWS = 256
-VIEW_MODES = {"color": 0, "normal": 1, "iterations": 2, "svo_depth": 3, "t": 4, "loops": 5, "phong": 6, "quality": 7}
+VIEW_MODES = {"color": 0, "normal": 1, "iterations": 2, "svo_depth": 3, "t": 4, "loops": 5, "phong": 6, "quality": 7,
+              "fill_svo_depth": 8, "fill_loops": 9, "fill_reflection": 10, "fill_shadow": 11}
+# the single-SVO renderer's otherParams.x (opencl/fillFrameBuffer_rgba.cl:633-700) as view modes
+FILL_VIEW_MODES = {0: "color", 1: "normal", 2: "fill_svo_depth", 3: "t", 4: "fill_loops", 5: "fill_reflection",
+                   6: "fill_shadow"}
 MISS = 0xFFFFFFFF
 BACKGROUND_RGBA = 0x334D33FF
 

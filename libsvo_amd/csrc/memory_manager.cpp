@@ -980,6 +980,13 @@ svo_status mm_trace_pools(const svo_mm* mm, int policy, TracePools* P) {
   P->fb_queue = mm->d_fb_queue;
   P->fb_state = mm->d_fb_state;
   P->fb_cap = P->num_gids;
+  if (mm->scene->has_package0) {
+    P->package0_normal = mm->scene->package0[0];
+    P->package0_color = mm->scene->package0[1];
+  } else if (!mm->scene->treelets.empty() && mm->scene->treelets[0].attribs.size() >= 2) {
+    P->package0_normal = mm->scene->treelets[0].attribs[0];
+    P->package0_color = mm->scene->treelets[0].attribs[1];
+  }
   if (policy == SVO_POLICY_TREELET) {
     // MAX_STACK_SIZE 13 / MAX_SVO_RAYCAST_DEPTH 12 (treeletRenderer_rgba.cl:7-8),
     // widened to max(13, D+1) so that leaves deeper than 12 are reachable.

@@ -37,6 +37,20 @@ constexpr uint32_t kMaxViews = 1u << 16;
 
 namespace svo_impl {
 
+svo_status check_view_mode(uint32_t mode, int policy, uint32_t i) {
+  if (mode > SVO_VIEW_FILL_SHADOW) { set_error("view %u: unknown view mode %u", i, mode); return SVO_ERR_ARG; }
+  if ((mode == SVO_VIEW_FILL_SHADOW || mode == SVO_VIEW_FILL_REFLECTION) && policy != SVO_POLICY_SINGLE) {
+    set_error("view %u: view mode %u traces single-SVO secondary rays and needs SVO_POLICY_SINGLE", i, mode);
+    return SVO_ERR_ARG;
+  }
+  return SVO_OK;
+}
+
+bool mode_needs_normals(uint32_t mode) {
+  return mode == SVO_VIEW_NORMAL || mode == SVO_VIEW_PHONG || mode == SVO_VIEW_FILL_SHADOW ||
+         mode == SVO_VIEW_FILL_REFLECTION;
+}
+
 svo_status stage_views_into(const svo_view* views, uint32_t n, int policy, const LaunchShape& shape, int device,
                             TraceFrame* frames, StageTotals* totals) {
   if (!views || n == 0 || n > kMaxViews) { set_error("need 1..%u views", kMaxViews); return SVO_ERR_ARG; }
@@ -60,7 +74,8 @@ svo_status stage_views_into(const svo_view* views, uint32_t n, int policy, const
     F.out = v.out;
     F.tile_begin = (uint32_t)tiles;
     F.tiles_x = (c.width + tw - 1) / tw;
-    if (c.view_mode > SVO_VIEW_QUALITY) { set_error("view %u: unknown view mode %u", i, c.view_mode); return SVO_ERR_ARG; }
+    r = check_view_mode(c.view_mode, policy, i);
+    if (r != SVO_OK) return r;
     if (v.feedback_stride > 65535u) { set_error("view %u: feedback stride %u too large", i, v.feedback_stride); return SVO_ERR_ARG; }
     F.fb_stride = v.feedback_stride;
     F.pad = 0;
@@ -74,7 +89,7 @@ svo_status stage_views_into(const svo_view* views, uint32_t n, int policy, const
 }
 
 bool view_needs_normals(const svo_view& v) {
-  return v.out.shade && (v.camera.view_mode == SVO_VIEW_NORMAL || v.camera.view_mode == SVO_VIEW_PHONG);
+  return v.out.shade && mode_needs_normals(v.camera.view_mode);
 }
 
 svo_status stage_views(const svo_view* views, uint32_t n, int policy, const LaunchShape& shape, int device,
@@ -568,7 +583,8 @@ extern "C" svo_status svo_batch_set_cameras(svo_batch* b, const svo_camera* cams
       set_error("camera %u changes the frame size of the batch", i);
       return SVO_ERR_ARG;
     }
-    if (cams[i].view_mode > SVO_VIEW_QUALITY) { set_error("camera %u: unknown view mode %u", i, cams[i].view_mode); return SVO_ERR_ARG; }
+    const svo_status r = check_view_mode(cams[i].view_mode, b->policy, i);
+    if (r != SVO_OK) return r;
   }
   HIP_TRY(hipSetDevice(b->device));
   if (b->pending) {
@@ -581,7 +597,7 @@ extern "C" svo_status svo_batch_set_cameras(svo_batch* b, const svo_camera* cams
   for (uint32_t i = 0; i < n; ++i) {
     b->h[i].cam = cams[i];
     b->sg.frames[i].cam = cams[i];
-    if (b->sg.frames[i].out.shade && (cams[i].view_mode == SVO_VIEW_NORMAL || cams[i].view_mode == SVO_VIEW_PHONG) &&
+    if (b->sg.frames[i].out.shade && mode_needs_normals(cams[i].view_mode) &&
         std::find(b->sg.normal_handles.begin(), b->sg.normal_handles.end(), b->mms[i]) == b->sg.normal_handles.end())
       b->sg.normal_handles.push_back(b->mms[i]);
   }

@@ -126,6 +126,11 @@ extern "C" svo_status svo_scene_open_single(const char* base, svo_scene** out) {
   sc->treelet_bytes = (uint32_t)tb;
   sc->nodes_per_treelet = (uint32_t)(tb / 8);
   sc->treelets.push_back(std::move(T));
+  if (np > 0) {
+    sc->has_package0 = true;
+    sc->package0[0] = packages[0];
+    sc->package0[1] = packages[1];
+  }
   std::string why;
   const int d = scene_depth(*sc, &why, &sc->inner_depth);
   if (d < 0) { set_error("%s: %s", p.c_str(), why.c_str()); return SVO_ERR_FORMAT; }

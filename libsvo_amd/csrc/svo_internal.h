@@ -76,6 +76,11 @@ struct svo_scene {
   // or a treelet root (an anchor leaf becomes one when its sub-treelet is
   // stitched in); max_depth - 1 in a well-formed tree
   uint32_t inner_depth = 0;
+  // attribute package 0 of a single-SVO file {packed_normal, packed_color}
+  // (the package the single-SVO renderer's reflection mode shades with);
+  // scenes from treelet bundles or builders use the root node's attributes
+  bool has_package0 = false;
+  uint32_t package0[2] = {0, 0};
   uint64_t tree_nodes = 0;
   uint64_t tree_leaves = 0;
   std::vector<svo_impl::TreeletData> treelets;

@@ -39,6 +39,10 @@ struct TracePools {
   uint32_t* fb_queue;            // treelets asked for in this cycle, first-come order
   uint32_t* fb_state;            // [0] entries appended to fb_queue (may exceed fb_cap), [1] stamp
   uint32_t fb_cap;
+  // attribute package 0 {packed_normal, packed_color}: what the single-SVO
+  // renderer's reflection mode shades every bounce with
+  uint32_t package0_normal;
+  uint32_t package0_color;
   uint32_t pad;
 };
 
@@ -67,7 +71,9 @@ bool shape_supported(const LaunchShape& s);
 // Enqueues the traversal of `num_tiles` tiles spread over `num_frames` views
 // (frames_dev in device memory) on `stream`; `stack_levels` is the largest
 // stack depth over the views; `shade` adds the view-mode pass for views
-// with a shade output.  Returns a hipError_t value (0 = success).
+// with a shade output (with policy SVO_POLICY_SINGLE that pass can trace the
+// single-SVO renderer's shadow and reflection rays, with an LDS stack of
+// `stack_levels`).  Returns a hipError_t value (0 = success).
 int launch_trace(const LaunchShape& shape, const TraceFrame* frames_dev, uint32_t num_frames, uint32_t num_tiles,
                  uint32_t stack_levels, int policy, bool shade, void* stream);
 
@@ -91,8 +97,13 @@ struct StageTotals {
 // Writes the descriptors of views[0..n) into frames[0..n) (no allocation).
 svo_status stage_views_into(const svo_view* views, uint32_t n, int policy, const LaunchShape& shape, int device,
                             TraceFrame* frames, StageTotals* totals);
-// The view shades with the normals pool (normal and Phong view modes).
+// The view shades with the normals pool (normal, Phong, and the single-SVO
+// renderer's shadow and reflection modes).
 bool view_needs_normals(const svo_view& v);
+bool mode_needs_normals(uint32_t mode);
+// A view mode the library knows, and the policy it needs (the shadow and
+// reflection modes trace single-SVO secondary rays).
+svo_status check_view_mode(uint32_t mode, int policy, uint32_t view_index);
 
 // Device scratch of the allocation-free launch: descriptors of at most
 // kNoAllocMaxViews views, in a fixed array that each GPU holds for the life

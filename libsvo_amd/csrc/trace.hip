@@ -787,12 +787,86 @@ __device__ float4 shade_pixel(uint32_t mode, const Hit& hit, const float d[3], c
         return make_float4(q, q, q, 1.0f);
       }
       return make_float4(0.6f, 0.3f, 0.3f, 1.0f);
+    case SVO_VIEW_FILL_SVO_DEPTH: {  // fillFrameBuffer_rgba.cl::shade_svoDepth (:482-489)
+      const float v = (float)hit.depth / 12.0f;
+      const float h = hit_ok ? 1.0f : 0.0f;
+      return make_float4(v * h, v * h, v * h, 1.0f);
+    }
+    case SVO_VIEW_FILL_LOOPS: {  // fillFrameBuffer_rgba.cl::shade_iterationDepth (:454-461)
+      const float v = 1.0f - (float)hit.loops / 300.0f;
+      return make_float4(v, v, v, 1.0f);
+    }
     default:  // SVO_VIEW_COLOR
       if (hit_ok) {
         const float4 c = get_color(S.colors[hit.node]);
         return make_float4(c.x, c.y, c.z, 1.0f);
       }
       return make_float4(0.2f, 0.3f, 0.2f, 1.0f);
+  }
+}
+
+// The single-SVO renderer's shadow and reflection modes
+// (fillFrameBuffer_rgba.cl:494-594), from the primary hit's node and centre:
+// every secondary ray is a policy-B sample() with tScaleRatio 0.001 from
+// nodeCenter + 0.01 * normal.  The shadow shading of a hit is
+// colour * nDotL * (1 - shadow) + colour * 0.2 (alpha 1); reflection blends
+// the shadow shadings of the hit and of up to two reflection hits with
+// strengths 0.75, 0.75^2, 0.75^3 and shades every bounce with attribute
+// package 0 (the reference's reflection mode never looks the attribute
+// index up).  One loop, one traversal per trip: the shadow ray of the
+// current hit, then (reflection) the ray to the next hit.
+template <typename Stack>
+__device__ float4 fill_secondary(uint32_t mode, const Hit& prim, const float d[3], const TracePools& S, Stack& stk) {
+  const float4 bg = make_float4(0.2f, 0.3f, 0.2f, 1.0f);
+  bool hit = prim.node != 0xFFFFFFFFu && prim.node < S.pool_nodes;
+  if (!hit) return bg;
+  const bool reflect = mode == SVO_VIEW_FILL_REFLECTION;
+  const float3 n = get_normal(reflect ? S.package0_normal : S.normals[prim.node]);
+  const float4 col = get_color(reflect ? S.package0_color : S.colors[prim.node]);
+  const float3 light = normalize3(make_float3(-0.5f, 1.0f, 0.6f));
+  float c[3] = {prim.center[0], prim.center[1], prim.center[2]};
+  float3 view = make_float3(d[0], d[1], d[2]);
+  float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  float strength = 0.75f;
+  uint32_t bounces_left = 2;
+  bool shadow_ray = true;
+  while (true) {
+    // origin nodeCenter + normal * 0.01f, direction towards the light or
+    // reflected about the normal
+    const float o[3] = {c[0] + (n.x * 0.01f), c[1] + (n.y * 0.01f), c[2] + (n.z * 0.01f)};
+    float3 dir = light;
+    if (!shadow_ray) {
+      const float k = dot3(n, view);
+      dir = normalize3(make_float3(view.x + ((-2.0f * n.x) * k), view.y + ((-2.0f * n.y) * k),
+                                   view.z + ((-2.0f * n.z) * k)));
+    }
+    float dd[3] = {dir.x, dir.y, dir.z};
+    const Hit h = trace_ray<1>(S, o, dd, 0.001f, stk);
+    const bool h_hit = h.node != 0xFFFFFFFFu && h.node < S.pool_nodes;
+    if (shadow_ray) {
+      const float ndl = cl_max(0.0f, dot3(n, light));
+      const float shadow = h_hit ? 1.0f : 0.0f;
+      const float4 sh = make_float4(((col.x * ndl) * (1.0f - shadow)) + (col.x * 0.2f),
+                                    ((col.y * ndl) * (1.0f - shadow)) + (col.y * 0.2f),
+                                    ((col.z * ndl) * (1.0f - shadow)) + (col.z * 0.2f), 1.0f);
+      if (!reflect) return sh;
+      acc = make_float4((acc.x * (1.0f - strength)) + (sh.x * strength), (acc.y * (1.0f - strength)) + (sh.y * strength),
+                        (acc.z * (1.0f - strength)) + (sh.z * strength), (acc.w * (1.0f - strength)) + (sh.w * strength));
+      strength *= 0.75f;
+      if (bounces_left == 0) return acc;
+      shadow_ray = false;
+    } else {
+      if (!h_hit) {  // the reflection ray left the scene: the background's turn
+        return make_float4((acc.x * (1.0f - strength)) + (bg.x * strength),
+                           (acc.y * (1.0f - strength)) + (bg.y * strength),
+                           (acc.z * (1.0f - strength)) + (bg.z * strength),
+                           (acc.w * (1.0f - strength)) + (bg.w * strength));
+      }
+      c[0] = h.center[0]; c[1] = h.center[1]; c[2] = h.center[2];
+      view = dir;
+      --bounces_left;
+      shadow_ray = true;
+    }
   }
 }
 
@@ -872,7 +946,8 @@ void trace_kernel(const TraceFrame* __restrict__ frames, uint32_t num_frames, ui
     // colour in place (keeps the shading out of this kernel's registers)
     uint4 raw;
     raw.x = hit.node;
-    if (F.cam.view_mode == SVO_VIEW_PHONG) {
+    if (F.cam.view_mode == SVO_VIEW_PHONG || F.cam.view_mode == SVO_VIEW_FILL_SHADOW ||
+        F.cam.view_mode == SVO_VIEW_FILL_REFLECTION) {
       raw.y = __float_as_uint(hit.center[0]);
       raw.z = __float_as_uint(hit.center[1]);
       raw.w = __float_as_uint(hit.center[2]);
@@ -887,9 +962,9 @@ void trace_kernel(const TraceFrame* __restrict__ frames, uint32_t num_frames, ui
 
 // Second pass of a launch with view-mode outputs: one lane per pixel of each
 // view whose shade output is set, tiles numbered as in trace_kernel.
-template <int TW>
+template <int TW, bool SECONDARY>
 __global__ __launch_bounds__(64) void shade_kernel(const TraceFrame* __restrict__ frames, uint32_t num_frames,
-                                                   uint32_t num_tiles) {
+                                                   uint32_t num_tiles, uint32_t stack_levels) {
   constexpr uint32_t TH = 64u / TW;
   const uint32_t tile = blockIdx.x;
   const uint32_t lane = threadIdx.x;
@@ -910,7 +985,9 @@ __global__ __launch_bounds__(64) void shade_kernel(const TraceFrame* __restrict_
   const uint4 raw = reinterpret_cast<const uint4*>(F.out.shade)[pix];
   Hit hit{};
   hit.node = raw.x;
-  if (F.cam.view_mode == SVO_VIEW_PHONG) {
+  const uint32_t mode = F.cam.view_mode;
+  const bool secondary = mode == SVO_VIEW_FILL_SHADOW || mode == SVO_VIEW_FILL_REFLECTION;
+  if (mode == SVO_VIEW_PHONG || secondary) {
     hit.center[0] = __uint_as_float(raw.y);
     hit.center[1] = __uint_as_float(raw.z);
     hit.center[2] = __uint_as_float(raw.w);
@@ -922,7 +999,21 @@ __global__ __launch_bounds__(64) void shade_kernel(const TraceFrame* __restrict_
   }
   float d[3];
   pixel_ray(F.cam, x, y, d);
-  reinterpret_cast<float4*>(F.out.shade)[pix] = shade_pixel(F.cam.view_mode, hit, d, F.pools);
+  float4 out;
+  if constexpr (SECONDARY) {
+    if (secondary) {
+      extern __shared__ uint32_t s_stack[];
+      RayStack<SVO_STACK_LDS, 64> stk;
+      stk.base = reinterpret_cast<uint2*>(s_stack);
+      stk.tid = lane;
+      out = fill_secondary(mode, hit, d, F.pools, stk);
+    } else {
+      out = shade_pixel(mode, hit, d, F.pools);
+    }
+  } else {
+    out = shade_pixel(mode, hit, d, F.pools);
+  }
+  reinterpret_cast<float4*>(F.out.shade)[pix] = out;
 }
 
 __global__ void ray_dir_kernel(svo_camera cam, float* dirs) {
@@ -1132,9 +1223,19 @@ int launch_trace(const LaunchShape& shape, const TraceFrame* frames_dev, uint32_
   int e = (int)hipGetLastError();
   if (e != 0 || !shade) return e;
   const dim3 sgrid(num_tiles), sblock(64);
-  if (shape.tile_w == 8) hipLaunchKernelGGL(shade_kernel<8>, sgrid, sblock, 0, (hipStream_t)stream, frames_dev, num_frames, num_tiles);
-  else if (shape.tile_w == 32) hipLaunchKernelGGL(shade_kernel<32>, sgrid, sblock, 0, (hipStream_t)stream, frames_dev, num_frames, num_tiles);
-  else hipLaunchKernelGGL(shade_kernel<16>, sgrid, sblock, 0, (hipStream_t)stream, frames_dev, num_frames, num_tiles);
+  hipStream_t st = (hipStream_t)stream;
+  if (policy == SVO_POLICY_SINGLE) {
+    // the pass that can also trace the shadow and reflection rays of the
+    // single-SVO renderer's view modes (a 64-lane LDS stack)
+    const size_t slds = (size_t)stack_levels * 64u * sizeof(uint2);
+    if (shape.tile_w == 8) hipLaunchKernelGGL((shade_kernel<8, true>), sgrid, sblock, slds, st, frames_dev, num_frames, num_tiles, stack_levels);
+    else if (shape.tile_w == 32) hipLaunchKernelGGL((shade_kernel<32, true>), sgrid, sblock, slds, st, frames_dev, num_frames, num_tiles, stack_levels);
+    else hipLaunchKernelGGL((shade_kernel<16, true>), sgrid, sblock, slds, st, frames_dev, num_frames, num_tiles, stack_levels);
+  } else {
+    if (shape.tile_w == 8) hipLaunchKernelGGL((shade_kernel<8, false>), sgrid, sblock, 0, st, frames_dev, num_frames, num_tiles, stack_levels);
+    else if (shape.tile_w == 32) hipLaunchKernelGGL((shade_kernel<32, false>), sgrid, sblock, 0, st, frames_dev, num_frames, num_tiles, stack_levels);
+    else hipLaunchKernelGGL((shade_kernel<16, false>), sgrid, sblock, 0, st, frames_dev, num_frames, num_tiles, stack_levels);
+  }
   return (int)hipGetLastError();
 }
 

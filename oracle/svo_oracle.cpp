@@ -726,6 +726,14 @@ void shade(uint32_t mode, const Result& r, bool hit, const float d[3], uint32_t 
       if (hit) out[0] = out[1] = out[2] = std::sqrt(std::sqrt(r.quality));
       else { out[0] = 0.6f; out[1] = 0.3f; out[2] = 0.3f; }
       return;
+    case 8: {  // fillFrameBuffer_rgba.cl::shade_svoDepth (:482-489): depth/12 * hit
+      const float v = (float)r.depth / 12.0f, h = hit ? 1.0f : 0.0f;
+      out[0] = out[1] = out[2] = v * h;
+      return;
+    }
+    case 9:  // fillFrameBuffer_rgba.cl::shade_iterationDepth (:454-461)
+      out[0] = out[1] = out[2] = 1.0f - (float)r.loops / 300.0f;
+      return;
     default: {
       if (hit) { float u[4]; unpack_rgba(color, u); out[0] = u[0] * k; out[1] = u[1] * k; out[2] = u[2] * k; }
       else { out[0] = 0.2f; out[1] = 0.3f; out[2] = 0.2f; }
@@ -934,6 +942,58 @@ int oracle_open_single(const char* base, oracle_single** out) {
 
 void oracle_free_single(oracle_single* s) { delete s; }
 
+// The single-SVO renderer's shadow (11) and reflection (10) modes,
+// fillFrameBuffer_rgba.cl:494-594, written as the reference's two nested
+// functions: shade_shadow is shade_diffuse_color_shadow (a policy-B sample
+// from nodeCenter + normal*0.01 towards normalize(-0.5, 1, 0.6) with
+// tScaleRatio 0.001; shadow = its hit as 0/1), shade_reflection is
+// shade_diffuse_color_reflection with numReflections 2, whose samples never
+// set attribIndex, so every bounce reads attribute package 0.
+void shade_shadow(const oracle_single* s, const Result& r, uint32_t attrib_pos, float out[4]) {
+  if (!r.hit) { out[0] = 0.2f; out[1] = 0.3f; out[2] = 0.2f; out[3] = 1.0f; return; }
+  const V3 n = get_normal(s->packages[2 * (size_t)attrib_pos]);
+  float u[4];
+  unpack_rgba(s->packages[2 * (size_t)attrib_pos + 1], u);
+  const float k = 0.003921569f;
+  const float color[4] = {u[0] * k, u[1] * k, u[2] * k, u[3] * k};
+  const V3 light = normalize3(V3{-0.5f, 1.0f, 0.6f});
+  const float ndl = cl_max(0.0f, dot3(n, light));
+  const float o[3] = {r.center[0] + n.x * 0.01f, r.center[1] + n.y * 0.01f, r.center[2] + n.z * 0.01f};
+  float d[3] = {light.x, light.y, light.z};
+  Result rs;
+  sample_fill(s->nodes.data(), s->nodes.size(), o, d, 0.001f, &rs);
+  const float shadow = rs.hit ? 1.0f : 0.0f;
+  for (int a = 0; a < 4; ++a) out[a] = color[a] * ndl * (1.0f - shadow) + color[a] * 0.2f;
+  out[3] = 1.0f;
+}
+
+void shade_reflection(const oracle_single* s, const Result& primary, const float ray_dir[3], float out[4]) {
+  if (!primary.hit) { out[0] = 0.2f; out[1] = 0.3f; out[2] = 0.2f; out[3] = 1.0f; return; }
+  Result cur = primary;
+  float result_color[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  V3 view{ray_dir[0], ray_dir[1], ray_dir[2]};
+  int reflect_counter = 2;
+  float strength = 0.75f;
+  while (true) {
+    const V3 normal = get_normal(s->packages[0]);
+    float sh[4];
+    shade_shadow(s, cur, 0, sh);
+    for (int a = 0; a < 4; ++a) result_color[a] = result_color[a] * (1.0f - strength) + sh[a] * strength;
+    strength *= 0.75f;
+    if (reflect_counter == 0 || !cur.hit) break;
+    const float dn = dot3(normal, view);
+    const V3 rd = normalize3(V3{view.x + -2.0f * normal.x * dn, view.y + -2.0f * normal.y * dn,
+                                view.z + -2.0f * normal.z * dn});
+    const float o[3] = {cur.center[0] + normal.x * 0.01f, cur.center[1] + normal.y * 0.01f,
+                        cur.center[2] + normal.z * 0.01f};
+    float d[3] = {rd.x, rd.y, rd.z};
+    sample_fill(s->nodes.data(), s->nodes.size(), o, d, 0.001f, &cur);
+    view = rd;
+    --reflect_counter;
+  }
+  for (int a = 0; a < 4; ++a) out[a] = result_color[a];
+}
+
 // Single-SVO frame with policy B (fillFrameBuffer_rgba.cl::renderToBuffer
 // mode 0): colour = packages[attribIndices[node]] (:640), alpha forced,
 // treelet id 0 for a hit (the whole SVO is one treelet).
@@ -962,7 +1022,9 @@ int oracle_render_single(const oracle_single* s, const void* camera, float* t, u
       if (pushes) pushes[p] = r.pushes;
       if (shade_out) {
         const size_t a = hit ? 2 * (size_t)s->aidx[r.node] : 0;
-        shade(cam.view_mode, r, hit, d0, hit ? s->packages[a + 1] : 0u, hit ? s->packages[a] : 0u, shade_out + 4 * p);
+        if (cam.view_mode == 11) shade_shadow(s, r, hit ? s->aidx[r.node] : 0u, shade_out + 4 * p);
+        else if (cam.view_mode == 10) shade_reflection(s, r, d0, shade_out + 4 * p);
+        else shade(cam.view_mode, r, hit, d0, hit ? s->packages[a + 1] : 0u, hit ? s->packages[a] : 0u, shade_out + 4 * p);
       }
     }
   }

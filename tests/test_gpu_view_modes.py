@@ -12,7 +12,7 @@ from libsvo_amd import workloads
 
 pytestmark = pytest.mark.gpu
 
-EXACT = ["color", "normal", "svo_depth", "t", "loops", "quality"]
+EXACT = ["color", "normal", "svo_depth", "t", "loops", "quality", "fill_svo_depth", "fill_loops"]
 
 
 @pytest.fixture(scope="module", autouse=True)
@@ -61,9 +61,14 @@ def test_view_mode_matches_oracle(scenes, mode, policy):
 def test_unknown_view_mode_rejected(scenes):
     mm, _ = scenes["menger"]
     cam = _cam("menger", 0, 0.0)
-    cam.view_mode = 8
+    cam.view_mode = 12
     with pytest.raises(svo.SvoError):
         svo.render(mm, cam, "treelet", shade=True)
+    # the shadow and reflection modes trace single-SVO rays: other policies refuse them
+    for mode in ("fill_shadow", "fill_reflection"):
+        for policy in ("treelet", "cpu3"):
+            with pytest.raises(svo.SvoError, match="SVO_POLICY_SINGLE"):
+                svo.render(mm, _cam("menger", mode, 0.0), policy, shade=True)
 
 
 def test_single_svo_policy_view_modes(tmp_path, menger8):
@@ -75,8 +80,9 @@ def test_single_svo_policy_view_modes(tmp_path, menger8):
     single = svo.Scene.open_single(base)
     mm = svo.TreeletMemoryManagerHip(single, workloads.incore_bytes_for(single), 0, resident_all=True)
     ora = oracle.OracleSingle(base)
-    for mode in EXACT:
+    for mode in EXACT + ["fill_shadow", "fill_reflection"]:
         cam = svo.with_view_mode(workloads.sphere_camera(9, 160, 120, lod=0.5), mode)
+        print(f"single-SVO policy, view mode {mode}", flush=True)
         g = svo.render(mm, cam, "single", shade=True)
         r = ora.render(cam, shade=True)
         same = (g["shade"].view(np.uint32) == r["shade"].view(np.uint32)) | (np.isnan(g["shade"]) & np.isnan(r["shade"]))
