@@ -446,10 +446,12 @@ __device__ __forceinline__ Uniform make_uniform(const TracePools& S) {
 //     exact, so t(p) = rc*p + mo is the value the reference computes for
 //     that plane however the coordinate was formed;
 //   * with rc finite, t(p) is monotone in p (rounding is monotone), so the
-//     compare-and-swap of the reference (swapToIncreasing) puts the plane on
-//     the near side of the ray into t0 and the far one into t1; where the two
-//     rounded values are equal they are the same non-zero number (a zero t
-//     needs the plane through the origin, the other plane is then non-zero);
+//     compare-and-swap of the reference (swapToIncreasing) puts the smaller
+//     of the two plane values into t0 and the larger into t1, and a min and
+//     a max compute them; where the two rounded values are equal they are
+//     the same number: t = -0 needs rc*p = -0 and mo = -o*rc = -0, so an
+//     origin coordinate of +-0 (|rc| >= 1 keeps o*rc from underflowing),
+//     and rays from such an origin take the generic loop;
 //   * a child box on one axis is bounded by the node's midplane and one face
 //     of the node, so a step evaluates two planes per axis; a pop evaluates
 //     only the parent's far faces (its tcMax).
@@ -457,9 +459,6 @@ template <int POLICY, typename Stack>
 __device__ Hit traverse_fast(const Uniform& U, const float o[3], const float d[3], const float rc[3], const float mo[3],
                              float t_min, float t_max, float ratio, Stack& stk, Hit r) {
   const float eps = policy_eps<POLICY>();
-  bool neg[3];
-#pragma unroll
-  for (int a = 0; a < 3; ++a) neg[a] = rc[a] < 0.0f;
 
   int scale = U.smax - 1;
   float c[3] = {0.0f, 0.0f, 0.0f};
@@ -553,12 +552,18 @@ __device__ Hit traverse_fast(const Uniform& U, const float o[3], const float d[3
         for (int a = 0; a < 3; ++a) {
           const float tmid = rc[a] * c[a] + mo[a];
           const float tface = rc[a] * (c[a] + (up[a] ? se : -se)) + mo[a];
-          const bool nearside = up[a] == neg[a];  // child on the near side of the midplane
-          t0[a] = nearside ? tface : tmid;
-          t1[a] = nearside ? tmid : tface;
+          t0[a] = fminf(tface, tmid);  // the near plane has the smaller t
+          t1[a] = fmaxf(tface, tmid);
         }
-        const float tc_min = cl_max(t0[0], cl_max(t0[1], t0[2]));
-        const float tc_max = cl_min(t1[0], cl_min(t1[1], t1[2]));
+        // IEEE max / min equal OpenCL's max / min here: no NaN (finite rc
+        // and planes), and no -0 (that needs the origin on a plane through
+        // 0, which takes the generic loop), so ties are equal bit patterns
+        float tc_min = fmaxf(t0[0], fmaxf(t0[1], t0[2]));
+        const float tc_max = fminf(t1[0], fminf(t1[1], t1[2]));
+        // keep tc_min next to the plane arithmetic: computed after the mask
+        // test its inputs would come from another block, and the compiler
+        // quiets such inputs before a max
+        asm("" : "+v"(tc_min));
         const uint32_t masks = (uint32_t)(nd >> 32);
         if ((masks >> (8u + ci)) & 1u) {
           const uint32_t child = nidx + (uint32_t)nd + (uint32_t)__popc((masks >> 8) & ((1u << ci) - 1u));  // relative
@@ -633,8 +638,8 @@ __device__ Hit traverse_fast(const Uniform& U, const float o[3], const float d[3
 }
 
 // Root-box test, direction clamp and reciprocals, then the fast loop when
-// every active lane of the wave has a finite reciprocal direction, else the
-// generic loop.
+// every active lane of the wave has a finite reciprocal direction and an
+// origin off the planes through 0, else the generic loop.
 template <int POLICY, typename Stack>
 __device__ Hit trace_ray(const TracePools& S, const float o[3], float d[3], float ratio, Stack& stk) {
   Hit r{0xFFFFFFFFu, __int_as_float(0x7F800000), 0u, 0u, 0.0f, 0u, 0u, {0.0f, 0.0f, 0.0f}, 0.0f};
@@ -648,7 +653,8 @@ __device__ Hit trace_ray(const TracePools& S, const float o[3], float d[3], floa
   for (int a = 0; a < 3; ++a) {
     rc[a] = 1.0f / dc[a];
     mo[a] = -o[a] * rc[a];
-    finite = finite && (fabsf(rc[a]) < __int_as_float(0x7F800000)) && (fabsf(mo[a]) < __int_as_float(0x7F800000));
+    finite = finite && (fabsf(rc[a]) < __int_as_float(0x7F800000)) && (fabsf(mo[a]) < __int_as_float(0x7F800000)) &&
+             o[a] != 0.0f;
   }
   if (__builtin_amdgcn_ballot_w64(!finite) == 0) {
     const Uniform U = make_uniform(S);
