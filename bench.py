@@ -110,6 +110,22 @@ def profiled_issue(shape):
     return None
 
 
+def stream_copy_gbs(stream, nbytes=1 << 30, iters=10):
+    """HBM bandwidth of a plain device copy (16 B per lane, grid-stride) of
+    `nbytes`, read plus written, over `iters` back-to-back copies after two
+    warm-ups: the measured counterpart of the 8 TB/s vendor peak."""
+    src, dst = svo.DeviceBuffer(nbytes), svo.DeviceBuffer(nbytes)
+    for _ in range(2):
+        svo._capi.check(svo.lib.svo_debug_copy(dst.ptr, src.ptr, nbytes, stream.handle), "svo_debug_copy")
+    svo._capi.check(svo.lib.svo_device_synchronize(), "sync")
+    t0 = time.perf_counter()
+    for _ in range(iters):
+        svo._capi.check(svo.lib.svo_debug_copy(dst.ptr, src.ptr, nbytes, stream.handle), "svo_debug_copy")
+    svo._capi.check(svo.lib.svo_device_synchronize(), "sync")
+    dt = time.perf_counter() - t0
+    return 2.0 * nbytes * iters / dt / 1e9
+
+
 def host_cpus():
     """CPU threads the baseline may use: the process's affinity mask, capped
     by OMP_NUM_THREADS when it is set (the CPU share granted to the process
@@ -666,6 +682,10 @@ def main():
             "issue": profiled_issue(shape),
         },
     }
+    copy_gbs = stream_copy_gbs(stream)
+    out["roofline"]["measured_copy_gbs"] = round(copy_gbs, 1)
+    out["roofline"]["frac_of_measured_copy"] = round(achieved / copy_gbs, 5)
+    log(f"stream copy {copy_gbs:.0f} GB/s (read + write); algorithmic rate {achieved / copy_gbs:.3f} of it")
     if not args.launch_shape:
         out["autotune"] = autotune
     # both policies' frames (untimed) for the CPU leg's pixel check

@@ -44,6 +44,14 @@ def max_threads() -> int:
     return _lib.oracle_max_threads()
 
 
+def data_point_color(value: float, vmin: float, vmax: float) -> np.ndarray:
+    """The iteration view mode's colour map at one value (FP32 RGB)."""
+    out = np.empty(3, np.float32)
+    _lib.oracle_data_point_color(ctypes.c_float(value), ctypes.c_float(vmin), ctypes.c_float(vmax),
+                                 out.ctypes.data_as(ctypes.c_void_p))
+    return out
+
+
 def ray_dirs(cam) -> np.ndarray:
     out = np.empty((cam.height, cam.width, 3), np.float32)
     if _lib.oracle_ray_dirs(ctypes.byref(cam), out.ctypes.data) != 0:

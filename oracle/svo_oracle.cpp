@@ -931,6 +931,13 @@ int oracle_ray_dirs(const void* camera, float* dirs) {
 
 int oracle_max_threads(void) { return omp_get_max_threads(); }
 
+// The iteration view mode's colour map (DataPointToColor over
+// WavelengthToRGB, treeletRenderer_rgba.cl:518-596) at one value, for tests.
+void oracle_data_point_color(float value, float vmin, float vmax, float* rgb) {
+  const V3 c = data_point_color(value, vmin, vmax);
+  rgb[0] = c.x; rgb[1] = c.y; rgb[2] = c.z;
+}
+
 int oracle_open_single(const char* base, oracle_single** out) {
   if (!base || !out) return 1;
   oracle_single* s = new oracle_single();

@@ -105,3 +105,58 @@ def test_phong(menger8):
     want = t * t * (F(3) - F(2) * t)
     assert np.allclose(r["shade"][hit][:, :3], want, rtol=0, atol=2e-6)
     assert np.all(r["shade"][~hit][:, :3] == np.array([0.4, 0.6, 0.4], np.float32))
+
+
+def test_svo_depth_is_depth_over_13_without_lod(menger8):
+    cam, r, _ = _render(menger8, "svo_depth")
+    hit = r["node"] != svo.MISS
+    # LOD off: every hit is a leaf at the sponge's depth 8
+    assert np.all(r["shade"][hit][:, :3] == F(8) / F(13))
+    assert np.all(r["shade"][~hit][:, :3] == 0.0)
+
+
+# Wavelength (nm) -> RGB of the published WavelengthToRGB algorithm (gamma
+# 0.8, intensity ramps below 420 and above 701 nm), as 8-bit values, at
+# wavelengths that the map reaches exactly from values v on [0, 400]
+# (w = 380 + v): the band edges 380 / 580 / 780 and points inside each band.
+WAVELENGTH_RGB = {
+    380: (97, 0, 97),     # violet end: r = b = 1 at 30 % intensity
+    430: (61, 0, 255),    # r = 1/6
+    455: (0, 97, 255),    # g = 0.3
+    480: (0, 213, 255),   # g = 0.8
+    505: (0, 255, 84),    # b = 0.25
+    530: (94, 255, 0),    # r = 2/7
+    580: (255, 255, 0),   # yellow
+    605: (255, 173, 0),   # g = 40/65
+    680: (255, 0, 0),
+    705: (248, 0, 0),     # intensity 0.3 + 0.7 * 75/79
+    780: (97, 0, 0),      # red end at 30 % intensity
+}
+
+
+def test_colour_map_matches_published_wavelength_values():
+    import oracle
+    for w, rgb in WAVELENGTH_RGB.items():
+        got = oracle.data_point_color(float(w - 380), 0.0, 400.0) * F(255)
+        assert np.array_equal(np.round(got).astype(int), np.array(rgb)), (w, got)
+    # outside [380, 780] nm the map is black
+    assert not np.any(oracle.data_point_color(-10.0, 0.0, 400.0))
+    assert not np.any(oracle.data_point_color(410.0, 0.0, 400.0))
+
+
+def test_phong_away_from_the_light_is_the_ambient_half(menger8):
+    """Where the normal faces away from the light (n.l <= 0), diffuse and
+    specular terms vanish: the pixel is smoothstep(colour / 2); everywhere
+    else it is no darker than that."""
+    cam, r, inc = _render(menger8, "phong")
+    hit = r["node"] != svo.MISS
+    col = _unpack(inc["colors"][r["node"][hit]])[:, :3].astype(np.float64) / 255.0
+    n = _unpack(inc["normals"][r["node"][hit]])[:, :3].astype(np.float64) / 255.0 * 2.0 - 1.0
+    lv = np.array([-120.0, 0.0, 0.0]) - r["center"][hit].astype(np.float64)
+    away = np.einsum("ij,ij->i", n, lv) < -1e-3 * np.linalg.norm(lv, axis=1)
+    base = np.clip(col * 0.5, 0, 1)
+    base = base * base * (3 - 2 * base)
+    got = r["shade"][hit][:, :3].astype(np.float64)
+    assert away.sum() > 50 and (~away).sum() > 50
+    assert np.allclose(got[away], base[away], atol=1e-5)
+    assert np.all(got >= base - 1e-5)
