@@ -114,6 +114,7 @@ def stream_copy_gbs(stream, nbytes=1 << 30, iters=10):
     """HBM bandwidth of a plain device copy (16 B per lane, grid-stride) of
     `nbytes`, read plus written, over `iters` back-to-back copies after two
     warm-ups: the measured counterpart of the 8 TB/s vendor peak."""
+    import libsvo_amd as svo
     src, dst = svo.DeviceBuffer(nbytes), svo.DeviceBuffer(nbytes)
     for _ in range(2):
         svo._capi.check(svo.lib.svo_debug_copy(dst.ptr, src.ptr, nbytes, stream.handle), "svo_debug_copy")
