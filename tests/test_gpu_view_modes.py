@@ -87,3 +87,50 @@ def test_single_svo_policy_view_modes(tmp_path, menger8):
         r = ora.render(cam, shade=True)
         same = (g["shade"].view(np.uint32) == r["shade"].view(np.uint32)) | (np.isnan(g["shade"]) & np.isnan(r["shade"]))
         assert same.all(), f"single {mode}: {int((~same).sum())} values differ"
+
+
+def test_single_svo_secondary_modes_in_a_replayed_batch(tmp_path, menger8):
+    """The shadow and reflection modes inside a hipGraph batch of the
+    single-SVO policy: replays, with eager work between them, and a camera
+    update that switches the views' modes, all equal to the oracle."""
+    import oracle
+    sc, _ = menger8
+    base = str(tmp_path / "m_v1b")
+    n = sc.treelet(0)[0]["num_nodes"]
+    sc.write_single(base, "v1", np.random.default_rng(5).permutation(n).astype(np.uint32))
+    single = svo.Scene.open_single(base)
+    mm = svo.TreeletMemoryManagerHip(single, workloads.incore_bytes_for(single), 0, resident_all=True)
+    ora = oracle.OracleSingle(base)
+    modes = ["fill_shadow", "fill_reflection", "color"]
+    cams = [svo.with_view_mode(workloads.sphere_camera(30 + i, 96 + 8 * i, 72, lod=0.25 * i), m)
+            for i, m in enumerate(modes)]
+    views = [svo.View(mm, c, svo.Frame.alloc(c.width, c.height, shade=True)) for c in cams]
+    r = svo.Renderer(0)
+    s = svo.Stream()
+    b = svo.Batch(r, views, "single", s)
+
+    def check(what):
+        for v in views:
+            g = v.frame.fetch()
+            ref = ora.render(v.camera, shade=True)
+            assert np.array_equal(g["node"], ref["node"]), what
+            same = g["shade"].view(np.uint32) == ref["shade"].view(np.uint32)
+            assert same.all(), f"{what}: {int((~same).sum())} values differ"
+
+    side = svo.View(mm, svo.with_view_mode(workloads.sphere_camera(3, 40, 30), "fill_shadow"),
+                    svo.Frame.alloc(40, 30, shade=True))
+    for rep in range(3):
+        r.render([side], "single", s)
+        print(f"secondary-mode batch replay {rep}", flush=True)
+        b.launch(s)
+        s.synchronize()
+        check(f"replay {rep}")
+    b.set_cameras([svo.with_view_mode(c, m) for c, m in zip(cams, ["fill_reflection", "fill_svo_depth", "fill_shadow"])], s)
+    b.launch(s)
+    s.synchronize()
+    check("after the mode switch")
+    # a batch of another policy refuses the secondary modes
+    t = svo.Batch(r, [svo.View(mm, svo.with_view_mode(cams[0], "color"), svo.Frame.alloc(96, 72, shade=True))],
+                  "treelet", s)
+    with pytest.raises(svo.SvoError, match="SVO_POLICY_SINGLE"):
+        t.set_cameras([svo.with_view_mode(cams[0], "fill_reflection")], s)
