@@ -173,21 +173,41 @@ __device__ __forceinline__ float node_tmax(const float c[3], float h, const floa
 
 // Per-ray stack: per level the node index of the parent left by a push and
 // that parent's tmax (the fast loop restores it on a pop instead of
-// recomputing it; the generic loop uses the index only).
+// recomputing it; the generic loop uses the index only).  The generic loop
+// addresses a level by its number; the fast loop, which holds a scale as the
+// bits of its half-edge (exponent field only, 2^23 apart from one scale to
+// the next), by those bits: in the LDS layouts ([level][lane] of 8-byte
+// entries, 8 * BLOCK bytes a level) the entry of scale bits `sb` sits at
+// byte address sb_base + (sb >> kShift), where bind_scales sets sb_base once
+// per ray so that the deepest level (min_push_scale) lands on this lane's
+// first entry.
+typedef __attribute__((address_space(3))) uint64_t lds_u64;  // an entry: low word first
+typedef __attribute__((address_space(3))) uint16_t lds_u16;
+__device__ __forceinline__ uint64_t entry(uint32_t lo, uint32_t hi) { return (uint64_t)lo | ((uint64_t)hi << 32); }
+template <int BLOCK>
+__host__ __device__ constexpr int lds_level_shift() { return 23 - 3 - (BLOCK == 64 ? 6 : BLOCK == 128 ? 7 : 8); }
+template <typename T>
+__device__ __forceinline__ uint32_t lds_address(const T* p) {
+  return (uint32_t)(size_t)(const __attribute__((address_space(3))) void*)p;
+}
 template <int STACK, int BLOCK>
 struct RayStack;
 template <int BLOCK>
 struct RayStack<SVO_STACK_LDS, BLOCK> {  // LDS, [level][lane] of 8-byte entries
   static constexpr bool kHoldsNodes = false;
+  static constexpr int kShift = lds_level_shift<BLOCK>();
   uint2* base;
   uint32_t tid;
+  uint32_t sb_base;
   __device__ __forceinline__ void put(int l, uint32_t v) { base[l * BLOCK + tid].x = v; }
   __device__ __forceinline__ uint32_t get(int l) const { return base[l * BLOCK + tid].x; }
-  __device__ __forceinline__ void put2(int l, uint32_t v, float t) { base[l * BLOCK + tid] = make_uint2(v, __float_as_uint(t)); }
-  __device__ __forceinline__ void get2(int l, uint32_t& v, float& t) const {
-    const uint2 e = base[l * BLOCK + tid];
-    v = e.x;
-    t = __uint_as_float(e.y);
+  __device__ __forceinline__ void bind_scales(uint32_t mps_sb) { sb_base = lds_address(base + tid) - (mps_sb >> kShift); }
+  __device__ __forceinline__ lds_u64* at(uint32_t sb) const { return (lds_u64*)(size_t)(sb_base + (sb >> kShift)); }
+  __device__ __forceinline__ void put2_sb(uint32_t sb, uint32_t v, float t) { *at(sb) = entry(v, __float_as_uint(t)); }
+  __device__ __forceinline__ void get2_sb(uint32_t sb, uint32_t& v, float& t) const {
+    const uint64_t e = *at(sb);
+    v = (uint32_t)e;
+    t = __uint_as_float((uint32_t)(e >> 32));
   }
 };
 // LDS, [level][lane]: 8-byte entries {node word, tmax} and, in a second
@@ -199,20 +219,26 @@ struct RayStack<SVO_STACK_LDS, BLOCK> {  // LDS, [level][lane] of 8-byte entries
 template <int BLOCK>
 struct RayStack<SVO_STACK_LDS_NODE, BLOCK> {
   static constexpr bool kHoldsNodes = true;
+  static constexpr int kShift = lds_level_shift<BLOCK>();
   uint2* base;
   uint16_t* masks;
   uint32_t tid;
   __device__ __forceinline__ void put(int l, uint32_t v) { base[l * BLOCK + tid].x = v; }
   __device__ __forceinline__ uint32_t get(int l) const { return base[l * BLOCK + tid].x; }
-  __device__ __forceinline__ void put3(int l, uint32_t cb, uint32_t m, float t) {
-    base[l * BLOCK + tid] = make_uint2(cb, __float_as_uint(t));
-    masks[l * BLOCK + tid] = (uint16_t)m;
+  uint32_t sb_base, sb_mbase;  // the mask array has 2-byte entries: 2 * BLOCK bytes a level
+  __device__ __forceinline__ void bind_scales(uint32_t mps_sb) {
+    sb_base = lds_address(base + tid) - (mps_sb >> kShift);
+    sb_mbase = lds_address(masks + tid) - (mps_sb >> (kShift + 2));
   }
-  __device__ __forceinline__ void get3(int l, uint32_t& cb, uint32_t& m, float& t) const {
-    const uint2 e = base[l * BLOCK + tid];
-    cb = e.x;
-    t = __uint_as_float(e.y);
-    m = masks[l * BLOCK + tid];
+  __device__ __forceinline__ void put3_sb(uint32_t sb, uint32_t cb, uint32_t m, float t) {
+    *(lds_u64*)(size_t)(sb_base + (sb >> kShift)) = entry(cb, __float_as_uint(t));
+    *(lds_u16*)(size_t)(sb_mbase + (sb >> (kShift + 2))) = (uint16_t)m;
+  }
+  __device__ __forceinline__ void get3_sb(uint32_t sb, uint32_t& cb, uint32_t& m, float& t) const {
+    const uint64_t e = *(const lds_u64*)(size_t)(sb_base + (sb >> kShift));
+    cb = (uint32_t)e;
+    t = __uint_as_float((uint32_t)(e >> 32));
+    m = *(const lds_u16*)(size_t)(sb_mbase + (sb >> (kShift + 2)));
   }
 };
 template <int BLOCK>
@@ -220,10 +246,20 @@ struct RayStack<SVO_STACK_PRIVATE, BLOCK> {  // private (scratch-backed) arrays
   static constexpr bool kHoldsNodes = false;
   uint32_t v[kMaxScale];
   float tm[kMaxScale];
+  uint32_t sb0;
   __device__ __forceinline__ void put(int l, uint32_t x) { v[l] = x; }
   __device__ __forceinline__ uint32_t get(int l) const { return v[l]; }
-  __device__ __forceinline__ void put2(int l, uint32_t x, float t) { v[l] = x; tm[l] = t; }
-  __device__ __forceinline__ void get2(int l, uint32_t& x, float& t) const { x = v[l]; t = tm[l]; }
+  __device__ __forceinline__ void bind_scales(uint32_t mps_sb) { sb0 = mps_sb; }
+  __device__ __forceinline__ void put2_sb(uint32_t sb, uint32_t x, float t) {
+    const uint32_t l = (sb - sb0) >> 23;
+    v[l] = x;
+    tm[l] = t;
+  }
+  __device__ __forceinline__ void get2_sb(uint32_t sb, uint32_t& x, float& t) const {
+    const uint32_t l = (sb - sb0) >> 23;
+    x = v[l];
+    t = tm[l];
+  }
 };
 
 // POLICY 0: treeletRenderer_rgba.cl::sample (:201-382).
@@ -412,14 +448,22 @@ __device__ Hit traverse(const TracePools& S, const float o[3], float d[3], float
 }
 
 // Wave-uniform traversal parameters, held in scalar registers.
+// Scales are held as the bits of the half-edge 2^(scale - scaleMax) of a
+// node at that scale (exponent field 127 + scale - scaleMax, mantissa 0):
+// positive floats order as their bits do, so scales compare as these bits.
 struct Uniform {
   const __attribute__((address_space(1))) uint64_t* nodes;
   int smax;
   int mps;
   uint32_t max_loops;
-  int lod_scale;  // policy A: scale whose inner nodes stop the descent (-1000 = none)
   uint32_t pool_nodes;
+  uint32_t mps_sb;   // min_push_scale: the deepest level of the stack
+  uint32_t lod_sb;   // policy A: the scale whose inner nodes stop the descent (0 = none)
+  uint32_t zero_sb;  // scale 0 (policy C stops there)
 };
+__device__ __forceinline__ uint32_t scale_bits(int scale, int smax) { return (uint32_t)(127 + scale - smax) << 23; }
+constexpr uint32_t kRootScaleBits = 0x3F000000u;  // scale scaleMax - 1: the root, half-edge 0.5
+constexpr uint32_t kScaleStep = 0x00800000u;      // one scale
 
 __device__ __forceinline__ uint32_t sgpr(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
 
@@ -432,8 +476,10 @@ __device__ __forceinline__ Uniform make_uniform(const TracePools& S) {
   U.mps = (int)sgpr(S.min_push_scale);
   U.max_loops = sgpr(S.max_loops);
   const uint32_t cap = sgpr(S.lod_cap);
-  U.lod_scale = cap != 0u ? U.smax - (int)cap : -1000;
   U.pool_nodes = sgpr(S.pool_nodes);
+  U.mps_sb = scale_bits(U.mps, U.smax);
+  U.lod_sb = (cap != 0u && cap < 127u) ? scale_bits(U.smax - (int)cap, U.smax) : 0u;
+  U.zero_sb = scale_bits(0, U.smax);
   return U;
 }
 
@@ -460,7 +506,9 @@ __device__ Hit traverse_fast(const Uniform& U, const float o[3], const float d[3
                              float t_min, float t_max, float ratio, Stack& stk, Hit r) {
   const float eps = policy_eps<POLICY>();
 
-  int scale = U.smax - 1;
+  // the current node's scale as the bits of its half-edge (see Uniform)
+  uint32_t sb = kRootScaleBits;
+  stk.bind_scales(U.mps_sb);
   float c[3] = {0.0f, 0.0f, 0.0f};
   float tmin = t_min, tmax = t_max;
   uint32_t nidx = 0;
@@ -487,9 +535,9 @@ __device__ Hit traverse_fast(const Uniform& U, const float o[3], const float d[3
     if (trip == next_check) {
       next_check += 8u;
       if ((trip & (trip - 1u)) == 0u) {
-        snap_iter = iter; snap_scale = (uint32_t)scale; snap_path = path;
+        snap_iter = iter; snap_scale = sb; snap_path = path;
         snap_tmin = __float_as_uint(tmin); snap_tmax = __float_as_uint(tmax); snap_pushes = pushes;
-      } else if (snap_iter != 0u && (uint32_t)scale == snap_scale && path == snap_path &&
+      } else if (snap_iter != 0u && sb == snap_scale && path == snap_path &&
                  __float_as_uint(tmin) == snap_tmin && __float_as_uint(tmax) == snap_tmax &&
                  iter + 2u < U.max_loops) {
         const uint32_t p = iter - snap_iter;
@@ -500,23 +548,23 @@ __device__ Hit traverse_fast(const Uniform& U, const float o[3], const float d[3
       }
     }
     // first half of a trip: one reference iteration if it pops
-    if (policy_pop<POLICY>(tmin, tmax)) {
+    bool pops = policy_pop<POLICY>(tmin, tmax);
+    if (pops) {
       ++iter;
       if (iter >= U.max_loops) {
         outcome = 3;
-      } else if (scale + 1 >= U.smax) {
+      } else if (sb >= kRootScaleBits) {
         outcome = 2;  // left the root
       } else {
-        const float se = exp2i(scale - U.smax);
-        ++scale;
+        const uint32_t seb = sb;  // half-edge of the node left
+        sb += kScaleStep;
         float tparent;
         uint32_t fidx;
         uint32_t pmask = 0;
-        if constexpr (Stack::kHoldsNodes) stk.get3(scale - U.mps, fidx, pmask, tparent);
-        else stk.get2(scale - U.mps, fidx, tparent);
+        if constexpr (Stack::kHoldsNodes) stk.get3_sb(sb, fidx, pmask, tparent);
+        else stk.get2_sb(sb, fidx, tparent);
         const uint32_t pl = (uint32_t)path;
         path >>= 3;
-        const uint32_t seb = __float_as_uint(se);
         // parent centre = child centre -/+ se (bit set: the child was the upper half)
         c[0] = c[0] + __uint_as_float(seb | ((pl << 29) & 0x80000000u));
         c[1] = c[1] + __uint_as_float(seb | ((pl << 30) & 0x80000000u));
@@ -532,15 +580,19 @@ __device__ Hit traverse_fast(const Uniform& U, const float o[3], const float d[3
         } else {
           nd = U.nodes[fidx];
         }
+        // only a lane that popped has a new interval to test again (the
+        // other lanes' test above still holds): the step half's test costs
+        // nothing on trips where no lane of the wave pops
+        pops = policy_pop<POLICY>(tmin, tmax);
       }
     }
     // second half: the next reference iteration if it steps into a child
-    if (outcome == 0 && !policy_pop<POLICY>(tmin, tmax)) {
+    if (outcome == 0 && !pops) {
       ++iter;
       if (iter >= U.max_loops) {
         outcome = 3;
       } else {
-        const float se = exp2i(scale - U.smax);  // half-edge of the current node
+        const float se = __uint_as_float(sb);  // half-edge of the current node
         const float te = tmin + eps;
         bool up[3];
         up[0] = ((o[0] + te * d[0]) - c[0]) > 0.0f;
@@ -568,9 +620,9 @@ __device__ Hit traverse_fast(const Uniform& U, const float o[3], const float d[3
         if ((masks >> (8u + ci)) & 1u) {
           const uint32_t child = nidx + (uint32_t)nd + (uint32_t)__popc((masks >> 8) & ((1u << ci) - 1u));  // relative
           bool stop;
-          if (POLICY == 0) stop = (ratio * tc_min > se) || scale == U.lod_scale;
+          if (POLICY == 0) stop = (ratio * tc_min > se) || sb == U.lod_sb;
           else if (POLICY == 1) stop = ratio * tc_max > se;
-          else stop = (ratio * tc_max > se) || scale == 0;
+          else stop = (ratio * tc_max > se) || sb == U.zero_sb;
           if (((masks >> ci) & 1u) || stop) {  // leaf, or inner node at the LOD / depth limit
             outcome = 1;
             hit_node = child;
@@ -578,15 +630,15 @@ __device__ Hit traverse_fast(const Uniform& U, const float o[3], const float d[3
             hit_kind = is_leaf ? 1u : 2u;
             hit_ci = ci;
             hit_err = is_leaf ? leaf_error(se, ratio, tc_min) : stop_error(se, ratio, tc_min);
-          } else if (child >= U.pool_nodes || scale < U.mps) {
+          } else if (child >= U.pool_nodes || sb < U.mps_sb) {
             outcome = 4;  // corrupt offset, or a push below the stack
           } else {
             ++pushes;
-            if constexpr (Stack::kHoldsNodes) stk.put3(scale - U.mps, nidx + (uint32_t)nd, masks, tmax);
-            else stk.put2(scale - U.mps, nidx, tmax);
+            if constexpr (Stack::kHoldsNodes) stk.put3_sb(sb, nidx + (uint32_t)nd, masks, tmax);
+            else stk.put2_sb(sb, nidx, tmax);
             path = (path << 3) | ci;
-            --scale;
-            const float h = se * 0.5f;
+            sb -= kScaleStep;
+            const float h = __uint_as_float(sb);  // the child's half-edge, se / 2
 #pragma unroll
             for (int a = 0; a < 3; ++a) c[a] = c[a] + (up[a] ? h : -h);
             tmin = tc_min;
@@ -603,7 +655,8 @@ __device__ Hit traverse_fast(const Uniform& U, const float o[3], const float d[3
 
   r.pushes = pushes;
   r.loops = iter;
-  const float se = exp2i(scale - U.smax);
+  const int scale = (int)(sb >> 23) - 127 + U.smax;
+  const float se = __uint_as_float(sb);
   if (outcome == 1) {
     r.node = hit_node;
     r.t = tmin;
