@@ -180,7 +180,9 @@ __device__ __forceinline__ float node_tmax(const float c[3], float h, const floa
 // entries, 8 * BLOCK bytes a level) the entry of scale bits `sb` sits at
 // byte address sb_base + (sb >> kShift), where bind_scales sets sb_base once
 // per ray so that the deepest level (min_push_scale) lands on this lane's
-// first entry.
+// first entry; the private layout indexes its arrays by the level
+// (sb - mps_sb) >> 23, from the deepest level's bits passed in (a scalar
+// register: a member would live in scratch beside the arrays).
 typedef __attribute__((address_space(3))) uint64_t lds_u64;  // an entry: low word first
 typedef __attribute__((address_space(3))) uint16_t lds_u16;
 __device__ __forceinline__ uint64_t entry(uint32_t lo, uint32_t hi) { return (uint64_t)lo | ((uint64_t)hi << 32); }
@@ -203,8 +205,8 @@ struct RayStack<SVO_STACK_LDS, BLOCK> {  // LDS, [level][lane] of 8-byte entries
   __device__ __forceinline__ uint32_t get(int l) const { return base[l * BLOCK + tid].x; }
   __device__ __forceinline__ void bind_scales(uint32_t mps_sb) { sb_base = lds_address(base + tid) - (mps_sb >> kShift); }
   __device__ __forceinline__ lds_u64* at(uint32_t sb) const { return (lds_u64*)(size_t)(sb_base + (sb >> kShift)); }
-  __device__ __forceinline__ void put2_sb(uint32_t sb, uint32_t v, float t) { *at(sb) = entry(v, __float_as_uint(t)); }
-  __device__ __forceinline__ void get2_sb(uint32_t sb, uint32_t& v, float& t) const {
+  __device__ __forceinline__ void put2_sb(uint32_t sb, uint32_t, uint32_t v, float t) { *at(sb) = entry(v, __float_as_uint(t)); }
+  __device__ __forceinline__ void get2_sb(uint32_t sb, uint32_t, uint32_t& v, float& t) const {
     const uint64_t e = *at(sb);
     v = (uint32_t)e;
     t = __uint_as_float((uint32_t)(e >> 32));
@@ -246,17 +248,16 @@ struct RayStack<SVO_STACK_PRIVATE, BLOCK> {  // private (scratch-backed) arrays
   static constexpr bool kHoldsNodes = false;
   uint32_t v[kMaxScale];
   float tm[kMaxScale];
-  uint32_t sb0;
   __device__ __forceinline__ void put(int l, uint32_t x) { v[l] = x; }
   __device__ __forceinline__ uint32_t get(int l) const { return v[l]; }
-  __device__ __forceinline__ void bind_scales(uint32_t mps_sb) { sb0 = mps_sb; }
-  __device__ __forceinline__ void put2_sb(uint32_t sb, uint32_t x, float t) {
-    const uint32_t l = (sb - sb0) >> 23;
+  __device__ __forceinline__ void bind_scales(uint32_t) {}
+  __device__ __forceinline__ void put2_sb(uint32_t sb, uint32_t mps_sb, uint32_t x, float t) {
+    const uint32_t l = (sb - mps_sb) >> 23;
     v[l] = x;
     tm[l] = t;
   }
-  __device__ __forceinline__ void get2_sb(uint32_t sb, uint32_t& x, float& t) const {
-    const uint32_t l = (sb - sb0) >> 23;
+  __device__ __forceinline__ void get2_sb(uint32_t sb, uint32_t mps_sb, uint32_t& x, float& t) const {
+    const uint32_t l = (sb - mps_sb) >> 23;
     x = v[l];
     t = tm[l];
   }
@@ -562,7 +563,7 @@ __device__ Hit traverse_fast(const Uniform& U, const float o[3], const float d[3
         uint32_t fidx;
         uint32_t pmask = 0;
         if constexpr (Stack::kHoldsNodes) stk.get3_sb(sb, fidx, pmask, tparent);
-        else stk.get2_sb(sb, fidx, tparent);
+        else stk.get2_sb(sb, U.mps_sb, fidx, tparent);
         const uint32_t pl = (uint32_t)path;
         path >>= 3;
         // parent centre = child centre -/+ se (bit set: the child was the upper half)
@@ -635,7 +636,7 @@ __device__ Hit traverse_fast(const Uniform& U, const float o[3], const float d[3
           } else {
             ++pushes;
             if constexpr (Stack::kHoldsNodes) stk.put3_sb(sb, nidx + (uint32_t)nd, masks, tmax);
-            else stk.put2_sb(sb, nidx, tmax);
+            else stk.put2_sb(sb, U.mps_sb, nidx, tmax);
             path = (path << 3) | ci;
             sb -= kScaleStep;
             const float h = __uint_as_float(sb);  // the child's half-edge, se / 2
