@@ -79,6 +79,28 @@ def profiled_traffic(shape):
     return None
 
 
+def profiled_l2_hit(shape):
+    """L2 hit rate TCC_HIT / (TCC_HIT + TCC_MISS) of the 1080p depth-11 frame
+    for a launch shape, from the same table of the committed profile (None
+    when the shape has no row there)."""
+    try:
+        text = open(PROFILE).read()
+    except OSError:
+        return None
+    key = f"| ({shape[0]}, {shape[1]}, {shape[2]}) |"
+    section = text.find("## HBM traffic per launch shape")
+    if section < 0:
+        return None
+    for line in text[section:].splitlines():
+        if line.startswith(key):
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            try:
+                return float(cells[4])
+            except (IndexError, ValueError):
+                return None
+    return None
+
+
 VALU_CEILING_PER_CU_CYCLE = 2.0  # gfx950: one wave64 VALU instruction per SIMD every 2 cycles, 4 SIMDs per CU
 
 
@@ -683,6 +705,11 @@ def main():
             "issue": profiled_issue(shape),
         },
     }
+    traffic = out["roofline"]["traffic"]
+    # the counters' HBM bytes per launch (profile) over this run's launch time,
+    # and the L2 hit rate of the same shape
+    out["roofline"]["hbm_gbs_measured"] = round(traffic / (ms / 1e3) / 1e9, 1) if traffic else None
+    out["roofline"]["l2_hit_rate"] = profiled_l2_hit(shape)
     copy_gbs = stream_copy_gbs(stream)
     out["roofline"]["measured_copy_gbs"] = round(copy_gbs, 1)
     out["roofline"]["frac_of_measured_copy"] = round(achieved / copy_gbs, 5)

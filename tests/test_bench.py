@@ -37,3 +37,15 @@ def test_profiled_issue_rows():
         assert 0.0 < r["simd_busy"] <= 1.0 and 0.0 < r["lane_use"] <= 1.0, (shape, r)
         assert abs(r["frac"] - r["achieved"] / 2.0) < 1e-3
     assert bench.profiled_issue((7, 7, "lds")) is None
+
+
+def test_profiled_l2_hit_rows():
+    """Every launch shape has an L2 hit rate in the traffic table, a fraction
+    in (0, 1] (bench.py reports the tuned shape's as roofline.l2_hit_rate)."""
+    sys.path.insert(0, ROOT)
+    import bench
+    import libsvo_amd as svo
+    for shape in svo.LAUNCH_SHAPES:
+        h = bench.profiled_l2_hit(shape)
+        assert h is not None and 0.0 < h <= 1.0, (shape, h)
+    assert bench.profiled_l2_hit((7, 7, "lds")) is None
