@@ -27,10 +27,11 @@
 // The loop is bound by VALU issue (several waves per SIMD take turns on one
 // VALU), so its cost is its dynamic instruction count.  Rays whose
 // reciprocal direction is finite on every axis (all but exactly axis-aligned
-// ones) run traverse_fast: plane parameters ordered by the sign of the
-// direction instead of compare-and-swap, two planes per axis per child step,
-// the parent's tmax restored from the stack on a pop, cycle checks scheduled
-// on the wave's (scalar) trip count, and a single loop exit (exits inside the
+// ones) and whose origin lies off the planes through 0 run traverse_fast:
+// two planes per axis per child step ordered by a min and a max instead of
+// compare-and-swap, scales held as the bits of their half-edge, the parent's
+// tmax restored from the stack on a pop, cycle checks scheduled on the
+// wave's (scalar) trip count, and a single loop exit (exits inside the
 // loop make the compiler copy the loop state at every exit).  One trip of
 // the fast loop is up to two of the reference's iterations: a pop, if the
 // ray pops, then a child step, if the ray does not pop again.  In a wave
@@ -501,7 +502,8 @@ __device__ __forceinline__ Uniform make_uniform(const TracePools& S) {
 //     and rays from such an origin take the generic loop;
 //   * a child box on one axis is bounded by the node's midplane and one face
 //     of the node, so a step evaluates two planes per axis; a pop evaluates
-//     only the parent's far faces (its tcMax).
+//     none (the parent's tmax comes back from the stack, its centre from the
+//     child's, exactly).
 template <int POLICY, typename Stack>
 __device__ Hit traverse_fast(const Uniform& U, const float o[3], const float d[3], const float rc[3], const float mo[3],
                              float t_min, float t_max, float ratio, Stack& stk, Hit r) {
